@@ -1,0 +1,158 @@
+"""fv3atm_amd -- MI355X-native FV3 tracer transport and vertical remap.
+
+Host-side mirror of the upstream Fortran interfaces of the path
+(GFDL_atmos_cubed_sphere: tp_core.F90 fv_tp_2d, fv_tracer2d.F90
+tracer_2d / tracer_2d_1L, fv_mapz.F90 map1_ppm / map_scalar / mapn_tracer /
+cs_profile / Lagrangian_to_Eulerian), calling the HIP kernels of
+libfv3atm_amd.so through its C-ABI (include/fv3atm_amd.h).  Same argument
+names and meanings; numpy arrays in the dycore's Fortran layout; failures
+raise RuntimeError, as mpp_error(FATAL) aborts upstream.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import FORM_FUSED32, FORM_FUSED64, FORM_PLAIN64, check, ptr
+
+NG = 3
+
+
+def _f64(a, name):
+    if a is None:
+        return None
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.f_contiguous):
+        raise TypeError(f"{name}: a Fortran-contiguous float64 array is required")
+    return a
+
+
+class Context:
+    """Device-resident domain: bounds, levels, tiles and grid metrics
+    (the gridstruct / bd / domain handles tracer_2d receives upstream)."""
+
+    def __init__(self, grids, npz: int, bd=None):
+        grids = list(grids)
+        n = grids[0].n
+        if bd is None:
+            bd = _ffi.Bounds(1, n, 1, n, 1 - NG, n + NG, 1 - NG, n + NG)
+        self.bd = bd
+        self.n = n
+        self.npx = self.npy = n + 1
+        self.npz = npz
+        self.ntiles = len(grids)
+        self.grids = grids
+        h = ctypes.c_void_p()
+        check(_ffi.lib().fv3_ctx_create(ctypes.byref(bd), self.npx, self.npy, npz, self.ntiles,
+                                        ctypes.byref(h)), "fv3_ctx_create")
+        self._h = h
+        for t, g in enumerate(grids, start=1):
+            gs = _ffi.Grid(*[ptr(_f64(getattr(g, k), k)) for k in
+                             ("area", "rarea", "dx", "dy", "dxa", "dya", "sin_sg", "del6_u",
+                              "del6_v")],
+                           g.da_min, int(g.sw_corner), int(g.se_corner), int(g.nw_corner),
+                           int(g.ne_corner), int(g.bounded_domain), g.grid_type)
+            check(_ffi.lib().fv3_ctx_set_grid(h, t, ctypes.byref(gs)), "fv3_ctx_set_grid")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _ffi.lib().fv3_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fv_tp_2d(ctx: Context, q, crx, cry, npx, npy, hord, fx, fy, xfx, yfx, ra_x, ra_y,
+             lim_fac=1.0, mfx=None, mfy=None, mass=None, nord=None, damp_c=None):
+    """tp_core.F90 fv_tp_2d: fills fx, fy; modifies q's corner halos."""
+    n = ctypes.c_int(nord) if nord is not None else None
+    dc = ctypes.c_double(damp_c) if damp_c is not None else None
+    check(_ffi.lib().fv3_fv_tp_2d(
+        ctx.handle, ptr(_f64(q, "q")), ptr(_f64(crx, "crx")), ptr(_f64(cry, "cry")), npx, npy,
+        hord, ptr(_f64(fx, "fx")), ptr(_f64(fy, "fy")), ptr(_f64(xfx, "xfx")),
+        ptr(_f64(yfx, "yfx")), ptr(_f64(ra_x, "ra_x")), ptr(_f64(ra_y, "ra_y")), lim_fac,
+        ptr(_f64(mfx, "mfx")), ptr(_f64(mfy, "mfy")), ptr(_f64(mass, "mass")),
+        ctypes.byref(n) if n is not None else None, ctypes.byref(dc) if dc is not None else None),
+        "fv_tp_2d")
+
+
+def _tracer(fn, name, ctx, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, hord, q_split, dt,
+            id_divg, nord_tr, trdm, lim_fac):
+    nsplit = np.zeros(npz, dtype=np.int32)
+    check(fn(ctx.handle, ptr(_f64(q, "q")), ptr(_f64(dp1, "dp1")), ptr(_f64(mfx, "mfx")),
+             ptr(_f64(mfy, "mfy")), ptr(_f64(cx, "cx")), ptr(_f64(cy, "cy")), npx, npy, npz, nq,
+             hord, q_split, dt, id_divg, nord_tr, trdm, lim_fac, ptr(nsplit)), name)
+    return nsplit
+
+
+def tracer_2d_1L(ctx: Context, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, hord, q_split=0,
+                 dt=1.0, id_divg=0, nord_tr=0, trdm=0.0, lim_fac=1.0):
+    """fv_tracer2d.F90 tracer_2d_1L (in place); returns nsplit(k)."""
+    return _tracer(_ffi.lib().fv3_tracer_2d_1l, "tracer_2d_1L", ctx, q, dp1, mfx, mfy, cx, cy,
+                   npx, npy, npz, nq, hord, q_split, dt, id_divg, nord_tr, trdm, lim_fac)
+
+
+def tracer_2d(ctx: Context, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, hord, q_split=0,
+              dt=1.0, id_divg=0, nord_tr=0, trdm=0.0, lim_fac=1.0):
+    """fv_tracer2d.F90 tracer_2d (in place); returns nsplit(k)."""
+    return _tracer(_ffi.lib().fv3_tracer_2d, "tracer_2d", ctx, q, dp1, mfx, mfy, cx, cy, npx, npy,
+                   npz, nq, hord, q_split, dt, id_divg, nord_tr, trdm, lim_fac)
+
+
+def map1_ppm(km, pe1, q1, qs, kn, pe2, q2, i1, i2, j, ibeg, iend, jbeg, jend, iv, kord):
+    """fv_mapz.F90 map1_ppm (row j)."""
+    check(_ffi.lib().fv3_map1_ppm(km, ptr(_f64(pe1, "pe1")), ptr(_f64(q1, "q1")),
+                                  ptr(_f64(qs, "qs")), kn, ptr(_f64(pe2, "pe2")),
+                                  ptr(_f64(q2, "q2")), i1, i2, j, ibeg, iend, jbeg, jend, iv,
+                                  kord), "map1_ppm")
+
+
+def map_scalar(km, pe1, q1, qs, kn, pe2, q2, i1, i2, j, ibeg, iend, jbeg, jend, iv, kord, q_min):
+    """fv_mapz.F90 map_scalar (row j)."""
+    check(_ffi.lib().fv3_map_scalar(km, ptr(_f64(pe1, "pe1")), ptr(_f64(q1, "q1")),
+                                    ptr(_f64(qs, "qs")), kn, ptr(_f64(pe2, "pe2")),
+                                    ptr(_f64(q2, "q2")), i1, i2, j, ibeg, iend, jbeg, jend, iv,
+                                    kord, q_min), "map_scalar")
+
+
+def mapn_tracer(nq, km, pe1, pe2, q1, dp2, kord, j, i1, i2, isd, ied, jsd, jed, q_min, fill):
+    """fv_mapz.F90 mapn_tracer (row j, q1 in place)."""
+    kord = np.ascontiguousarray(kord, dtype=np.int32)
+    check(_ffi.lib().fv3_mapn_tracer(nq, km, ptr(_f64(pe1, "pe1")), ptr(_f64(pe2, "pe2")),
+                                     ptr(_f64(q1, "q1")), ptr(_f64(dp2, "dp2")), ptr(kord), j,
+                                     i1, i2, isd, ied, jsd, jed, q_min, int(fill)), "mapn_tracer")
+
+
+def cs_profile(qs, a4, delp, km, i1, i2, iv, kord):
+    """fv_mapz.F90 cs_profile (a4 in place)."""
+    check(_ffi.lib().fv3_cs_profile(ptr(_f64(qs, "qs")), ptr(_f64(a4, "a4")),
+                                    ptr(_f64(delp, "delp")), km, i1, i2, iv, kord), "cs_profile")
+
+
+def lagrangian_to_eulerian_tracers(ctx: Context, km, nq, ak, bk, ptop, delp, q, pt, kord_tr,
+                                   kord_tm, t_min=184.0, fill=False):
+    """Tracer/pt subset of fv_mapz.F90 Lagrangian_to_Eulerian (in place)."""
+    kord_tr = np.ascontiguousarray(np.broadcast_to(kord_tr, (nq,)), dtype=np.int32)
+    check(_ffi.lib().fv3_lagrangian_to_eulerian_tracers(
+        ctx.handle, km, nq, ptr(np.ascontiguousarray(ak, dtype=np.float64)),
+        ptr(np.ascontiguousarray(bk, dtype=np.float64)), ptop, ptr(_f64(delp, "delp")),
+        ptr(_f64(q, "q")), ptr(_f64(pt, "pt")), ptr(kord_tr), kord_tm, t_min, int(fill)),
+        "Lagrangian_to_Eulerian")
+
+
+def step_dev(ctx: Context, form, nq, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx, mfy, ak, bk,
+             ptop, hord=8, kord_tr=9, kord_tm=10, t_min=184.0, nsplit_cap=0, stream=None):
+    """Device-resident advection + remap step on torch device tensors."""
+    check(_ffi.lib().fv3_step_dev(ctx.handle, form, nq, ptr(q_in), ptr(q_out), ptr(pt), ptr(dp1),
+                                  ptr(delp_out), ptr(cx), ptr(cy), ptr(mfx), ptr(mfy), ptr(ak),
+                                  ptr(bk), ptop, hord, kord_tr, kord_tm, t_min, nsplit_cap,
+                                  stream), "fv3_step_dev")

@@ -1,0 +1,80 @@
+"""ctypes binding of the C-ABI declared in include/fv3atm_amd.h.
+
+The product path has no CPU fallback: if libfv3atm_amd.so is missing or a
+call fails, a RuntimeError is raised with the library's own error text.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_here = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_here, "libfv3atm_amd.so")
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+IP = ctypes.POINTER(ctypes.c_int)
+DP = ctypes.POINTER(ctypes.c_double)
+
+FORM_PLAIN64, FORM_FUSED64, FORM_FUSED32 = 0, 1, 2
+
+
+class Bounds(ctypes.Structure):
+    _fields_ = [(k, I) for k in ("is_", "ie", "js", "je", "isd", "ied", "jsd", "jed")]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [(k, P) for k in ("area", "rarea", "dx", "dy", "dxa", "dya", "sin_sg",
+                                 "del6_u", "del6_v")] + [("da_min", D)] + \
+               [(k, I) for k in ("sw_corner", "se_corner", "nw_corner", "ne_corner",
+                                 "bounded_domain", "grid_type")]
+
+
+# every symbol the header declares, with its ctypes signature
+SIGNATURES = {
+    "fv3_last_error": (ctypes.c_char_p, []),
+    "fv3_ctx_create": (I, [ctypes.POINTER(Bounds), I, I, I, I, ctypes.POINTER(P)]),
+    "fv3_ctx_destroy": (I, [P]),
+    "fv3_ctx_set_grid": (I, [P, I, ctypes.POINTER(Grid)]),
+    "fv3_fv_tp_2d": (I, [P, P, P, P, I, I, I, P, P, P, P, P, P, D, P, P, P, IP, DP]),
+    "fv3_tracer_2d_1l": (I, [P, P, P, P, P, P, P, I, I, I, I, I, I, D, I, I, D, D, P]),
+    "fv3_tracer_2d": (I, [P, P, P, P, P, P, P, I, I, I, I, I, I, D, I, I, D, D, P]),
+    "fv3_map1_ppm": (I, [I, P, P, P, I, P, P, I, I, I, I, I, I, I, I, I]),
+    "fv3_map_scalar": (I, [I, P, P, P, I, P, P, I, I, I, I, I, I, I, I, I, D]),
+    "fv3_mapn_tracer": (I, [I, I, P, P, P, P, P, I, I, I, I, I, I, I, D, I]),
+    "fv3_cs_profile": (I, [P, P, P, I, I, I, I, I]),
+    "fv3_lagrangian_to_eulerian_tracers": (I, [P, I, I, P, P, D, P, P, P, P, I, D, I]),
+    "fv3_step_dev": (I, [P, I, I, P, P, P, P, P, P, P, P, P, P, P, D, I, I, I, D, I, P]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().fv3_last_error()
+        raise RuntimeError(f"{what} failed: {msg.decode() if msg else rc}")
+
+
+def ptr(a):
+    """Address of a numpy array (host) or torch tensor (device); None -> NULL."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)

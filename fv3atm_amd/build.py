@@ -1,0 +1,110 @@
+"""Build the native parts in-tree: libfv3atm_amd.so (HIP, gfx950),
+libfv3_cases.so (host C) and the Fortran interface layer (amdflang).
+
+Usage: python -m fv3atm_amd.build
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+FC = os.path.join(ROCM, "bin", "amdflang")
+ARCH = "gfx950"
+
+# contraction off everywhere except the FP32 fused kernels; packed FP32 math off
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HIP_SOURCES = {
+    "tp_plain.hip": ["-ffp-contract=off"],
+    "mapz_plain.hip": ["-ffp-contract=off"],
+    "capi.hip": ["-ffp-contract=off"],
+    "fused.hip": ["-ffp-contract=off"],
+    "fused32.hip": ["-ffp-contract=fast"],
+}
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops' is not a recognized" not in l
+                    and "warning: feature flag" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build_hip():
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+
+    def one(s):
+        obj = os.path.join(OBJ, s.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, *HIP_SOURCES[s], "-c", os.path.join(CSRC, s), "-o", obj])
+        return obj
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(one, srcs))
+    out = os.path.join(HERE, "libfv3atm_amd.so")
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", out + ".tmp", *objs])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_cases():
+    out = os.path.join(HERE, "libfv3_cases.so")
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-o", out + ".tmp",
+          os.path.join(CSRC, "test_cases.c"), "-lm"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_fortran():
+    """The ISO_C_BINDING layer (Fortran interfaces of the path) and its
+    harness executable; skipped when amdflang is absent."""
+    fdir = os.path.join(HERE, "fortran")
+    srcs = ["fv_arrays_stub.F90", "fv3atm_amd_cabi.F90", "tp_core_mod.F90", "fv_tracer2d_mod.F90",
+            "fv_mapz_mod.F90"]
+    if not os.path.exists(FC) or not all(os.path.exists(os.path.join(fdir, s)) for s in srcs):
+        return None
+    mod = os.path.join(OBJ, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    lib = os.path.join(HERE, "libfv3atm_amd_f.so")
+    _run([FC, "-O2", "-fPIC", "-shared", "-module-dir", mod, "-o", lib + ".tmp",
+          *[os.path.join(fdir, s) for s in srcs], "-L" + HERE, "-lfv3atm_amd",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(lib + ".tmp", lib)
+    harness = os.path.join(fdir, "harness.F90")
+    if os.path.exists(harness):
+        exe = os.path.join(HERE, "fv3_harness")
+        _run([FC, "-O2", "-I" + mod, "-o", exe + ".tmp", harness, "-L" + HERE, "-lfv3atm_amd_f",
+              "-lfv3atm_amd", "-Wl,-rpath,$ORIGIN"])
+        os.replace(exe + ".tmp", exe)
+    return lib
+
+
+def build_oracle():
+    sh = os.path.join(ROOT, "oracle", "build_oracle.sh")
+    if os.path.exists(sh):
+        _run(["sh", sh])
+
+
+def build_all(verbose=True):
+    steps = [("cases", build_cases), ("oracle", build_oracle), ("hip", build_hip),
+             ("fortran", build_fortran)]
+    for name, fn in steps:
+        r = fn()
+        if verbose:
+            print(f"[build] {name}: {r or 'ok'}", flush=True)
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,636 @@
+// capi.hip -- the extern "C" boundary of libfv3atm_amd.so: context, metric
+// upload, host-array entry points mirroring the upstream Fortran interfaces
+// (staged through device buffers), and the device-resident step.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/fv3atm_amd.h"
+#include "fused.h"
+#include "fv3_internal.h"
+#include "mapz_column.h"
+
+struct fv3_ctx {
+  fv3::Ctx c;
+};
+
+namespace fv3 {
+
+static thread_local char g_err[1024] = "";
+
+int set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return 1;
+}
+
+Dims make_dims(int is, int ie, int js, int je, int isd, int ied, int jsd, int jed, int npx,
+               int npy, unsigned corners, bool bounded, int grid_type) {
+  Dims d{};
+  d.is = is;
+  d.ie = ie;
+  d.js = js;
+  d.je = je;
+  d.isd = isd;
+  d.ied = ied;
+  d.jsd = jsd;
+  d.jed = jed;
+  d.npx = npx;
+  d.npy = npy;
+  d.nid = ied - isd + 1;
+  d.njd = jed - jsd + 1;
+  d.nix = ie - is + 2;
+  d.niy = ie - is + 1;
+  d.nj = je - js + 1;
+  d.njy = je - js + 2;
+  d.s2 = (long long)d.nid * d.njd;
+  d.sx = (long long)d.nix * d.njd;
+  d.sy = (long long)d.nid * d.njy;
+  d.smx = (long long)d.nix * d.nj;
+  d.smy = (long long)d.niy * d.njy;
+  d.srx = (long long)d.niy * d.njd;
+  d.sry = (long long)d.nid * d.nj;
+  d.sdp = (long long)d.niy * d.nj;
+  d.corners = corners;
+  d.bounded = bounded;
+  d.edge = !bounded && grid_type < 3;
+  return d;
+}
+
+void *ctx_scratch(Ctx &c, size_t bytes) {
+  if (bytes > c.scratch_bytes) {
+    if (c.scratch) {
+      (void)hipStreamSynchronize(c.stream);
+      (void)hipFree(c.scratch);
+      c.scratch = nullptr;
+      c.scratch_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&c.scratch, bytes);
+    if (e != hipSuccess) {
+      set_error("scratch allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+      c.scratch = nullptr;
+      return nullptr;
+    }
+    c.scratch_bytes = bytes;
+  }
+  return c.scratch;
+}
+
+void *ctx_stage(Ctx &c, size_t bytes) {
+  if (bytes > c.stage_bytes) {
+    if (c.stage) {
+      (void)hipStreamSynchronize(c.stream);
+      (void)hipFree(c.stage);
+      c.stage = nullptr;
+      c.stage_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&c.stage, bytes);
+    if (e != hipSuccess) {
+      set_error("staging allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+      c.stage = nullptr;
+      return nullptr;
+    }
+    c.stage_bytes = bytes;
+  }
+  return c.stage;
+}
+
+template <>
+const GridDev<double> &grid_of<double>(const Ctx &c) {
+  return c.grid64;
+}
+template <>
+const GridDev<float> &grid_of<float>(const Ctx &c) {
+  return c.grid32;
+}
+
+// FMS cubic-mosaic contacts: the source cell of every edge-halo cell.
+static void build_halo_table(int n, std::vector<long long> &dst, std::vector<long long> &src) {
+  const int isd = 1 - NG, nid = n + 2 * NG;
+  auto cell = [&](int i, int j) { return (long long)(j - isd) * nid + (i - isd); };
+  auto nxt = [](int t, int s) { return ((t - 1 + s) % 6 + 6) % 6 + 1; };
+  auto add = [&](int t, int i, int j, int ts, int si, int sj) {
+    dst.push_back(((long long)(t - 1) << 40) | cell(i, j));
+    src.push_back(((long long)(ts - 1) << 40) | cell(si, sj));
+  };
+  for (int t = 1; t <= 6; t++) {
+    const bool odd = t % 2 == 1;
+    for (int e = 1; e <= NG; e++)
+      for (int a = 1; a <= n; a++) {
+        if (odd) {
+          add(t, n + e, a, nxt(t, 1), e, a);
+          add(t, a, n + e, nxt(t, 2), e, n + 1 - a);
+          add(t, 1 - e, a, nxt(t, -2), n + 1 - a, n + 1 - e);
+          add(t, a, 1 - e, nxt(t, -1), a, n + 1 - e);
+        } else {
+          add(t, n + e, a, nxt(t, 2), n + 1 - a, e);
+          add(t, a, n + e, nxt(t, 1), a, e);
+          add(t, 1 - e, a, nxt(t, -1), n + 1 - e, a);
+          add(t, a, 1 - e, nxt(t, -2), n + 1 - e, n + 1 - a);
+        }
+      }
+  }
+}
+
+// default context for the stateless row entry points (map1_ppm & co.)
+static Ctx *row_ctx() {
+  static std::once_flag once;
+  static Ctx *c = nullptr;
+  std::call_once(once, [] {
+    c = new Ctx();
+    (void)hipStreamCreate(&c->stream);
+  });
+  return c;
+}
+
+}  // namespace fv3
+
+using namespace fv3;
+
+extern "C" {
+
+const char *fv3_last_error(void) { return fv3::g_err; }
+
+int fv3_ctx_create(const fv3_bounds *bd, int npx, int npy, int npz, int ntiles, fv3_ctx **out) {
+  if (!bd || !out) return set_error("fv3_ctx_create: NULL argument");
+  *out = nullptr;
+  if (ntiles != 1 && ntiles != 6) return set_error("ntiles must be 1 or 6 (got %d)", ntiles);
+  if (npz < 1) return set_error("npz must be >= 1 (got %d)", npz);
+  if (bd->is - bd->isd != NG || bd->ied - bd->ie != NG || bd->js - bd->jsd != NG || bd->jed - bd->je != NG)
+    return set_error("bounds must carry a %d-cell halo", NG);
+  if (bd->ie - bd->is + 1 < 8 || bd->je - bd->js + 1 < 8)
+    return set_error("compute domain must be at least 8x8");
+  if (ntiles == 6 && (bd->is != 1 || bd->js != 1 || bd->ie != npx - 1 || bd->je != npy - 1 || npx != npy))
+    return set_error("a six-tile context needs whole square tiles (is=js=1, ie=je=npx-1)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error("no HIP device visible");
+  auto *h = new fv3_ctx();
+  Ctx &c = h->c;
+  c.npz = npz;
+  c.ntiles = ntiles;
+  (void)hipGetDevice(&c.dev);
+  c.d = make_dims(bd->is, bd->ie, bd->js, bd->je, bd->isd, bd->ied, bd->jsd, bd->jed, npx, npy, 15u,
+                  false, 0);
+  if (hipStreamCreate(&c.stream) != hipSuccess) {
+    delete h;
+    return set_error("hipStreamCreate failed");
+  }
+  const long long s2 = c.d.s2, sdx = (long long)c.d.nid * (c.d.njd + 1), sdy = (long long)(c.d.nid + 1) * c.d.njd;
+  const long long per_tile = s2 * 4 + sdx * 2 + sdy * 2 + s2 * 9;
+  if (hipMalloc(&c.g64, sizeof(double) * per_tile * ntiles) != hipSuccess ||
+      hipMalloc(&c.g32, sizeof(float) * per_tile * ntiles) != hipSuccess) {
+    fv3_ctx_destroy(h);
+    return set_error("metric allocation failed");
+  }
+  auto lay = [&](auto *base, auto &gd) {
+    using T = std::remove_pointer_t<decltype(base)>;
+    T *p = base;
+    gd.area = p;
+    p += s2 * ntiles;
+    gd.rarea = p;
+    p += s2 * ntiles;
+    gd.dxa = p;
+    p += s2 * ntiles;
+    gd.dya = p;
+    p += s2 * ntiles;
+    gd.dx = p;
+    p += sdx * ntiles;
+    gd.del6_u = p;
+    p += sdx * ntiles;
+    gd.dy = p;
+    p += sdy * ntiles;
+    gd.del6_v = p;
+    p += sdy * ntiles;
+    gd.sin_sg = p;
+    gd.s2 = s2;
+    gd.sdx = sdx;
+    gd.sdy = sdy;
+  };
+  lay(c.g64, c.grid64);
+  lay(c.g32, c.grid32);
+  if (ntiles == 6) {
+    std::vector<long long> dst, src;
+    build_halo_table(npx - 1, dst, src);
+    c.halo_n = (int)dst.size();
+    if (hipMalloc(&c.halo_dst, sizeof(long long) * dst.size()) != hipSuccess ||
+        hipMalloc(&c.halo_src, sizeof(long long) * src.size()) != hipSuccess) {
+      fv3_ctx_destroy(h);
+      return set_error("halo table allocation failed");
+    }
+    (void)hipMemcpy(c.halo_dst, dst.data(), sizeof(long long) * dst.size(), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c.halo_src, src.data(), sizeof(long long) * src.size(), hipMemcpyHostToDevice);
+  }
+  *out = h;
+  return 0;
+}
+
+int fv3_ctx_destroy(fv3_ctx *h) {
+  if (!h) return 0;
+  Ctx &c = h->c;
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  (void)hipFree(c.g64);
+  (void)hipFree(c.g32);
+  (void)hipFree(c.halo_dst);
+  (void)hipFree(c.halo_src);
+  (void)hipFree(c.scratch);
+  (void)hipFree(c.stage);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete h;
+  return 0;
+}
+
+int fv3_ctx_set_grid(fv3_ctx *h, int tile, const fv3_grid *g) {
+  if (!h || !g) return set_error("fv3_ctx_set_grid: NULL argument");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (tile < 1 || tile > c.ntiles) return set_error("tile %d out of 1..%d", tile, c.ntiles);
+  const long long s2 = c.d.s2, sdx = c.grid64.sdx, sdy = c.grid64.sdy;
+  const int t = tile - 1;
+  struct F {
+    const double *src;
+    double *d64;
+    float *d32;
+    long long n;
+  } f[] = {
+      {g->area, (double *)c.grid64.area + t * s2, (float *)c.grid32.area + t * s2, s2},
+      {g->rarea, (double *)c.grid64.rarea + t * s2, (float *)c.grid32.rarea + t * s2, s2},
+      {g->dxa, (double *)c.grid64.dxa + t * s2, (float *)c.grid32.dxa + t * s2, s2},
+      {g->dya, (double *)c.grid64.dya + t * s2, (float *)c.grid32.dya + t * s2, s2},
+      {g->dx, (double *)c.grid64.dx + t * sdx, (float *)c.grid32.dx + t * sdx, sdx},
+      {g->del6_u, (double *)c.grid64.del6_u + t * sdx, (float *)c.grid32.del6_u + t * sdx, sdx},
+      {g->dy, (double *)c.grid64.dy + t * sdy, (float *)c.grid32.dy + t * sdy, sdy},
+      {g->del6_v, (double *)c.grid64.del6_v + t * sdy, (float *)c.grid32.del6_v + t * sdy, sdy},
+      {g->sin_sg, (double *)c.grid64.sin_sg + t * 9 * s2, (float *)c.grid32.sin_sg + t * 9 * s2, 9 * s2},
+  };
+  std::vector<float> tmp;
+  for (auto &x : f) {
+    if (!x.src) return set_error("fv3_ctx_set_grid: a metric array is NULL");
+    FV3_HIP(hipMemcpy(x.d64, x.src, sizeof(double) * x.n, hipMemcpyHostToDevice));
+    tmp.resize(x.n);
+    for (long long k = 0; k < x.n; k++) tmp[k] = (float)x.src[k];
+    FV3_HIP(hipMemcpy(x.d32, tmp.data(), sizeof(float) * x.n, hipMemcpyHostToDevice));
+  }
+  if (tile == 1 || g->da_min < c.da_min || c.da_min == 0.0) c.da_min = g->da_min;
+  unsigned corners = (g->sw_corner ? 1u : 0u) | (g->se_corner ? 2u : 0u) | (g->ne_corner ? 4u : 0u) |
+                     (g->nw_corner ? 8u : 0u);
+  c.d = make_dims(c.d.is, c.d.ie, c.d.js, c.d.je, c.d.isd, c.d.ied, c.d.jsd, c.d.jed, c.d.npx, c.d.npy,
+                  corners, g->bounded_domain != 0, g->grid_type);
+  c.grid_set = true;
+  return 0;
+}
+
+// ---- host-array entry points --------------------------------------------------
+struct HostArr {
+  void *host;
+  size_t bytes;
+  bool in, out;
+};
+
+static int stage_run(Ctx &c, std::vector<HostArr> arrs, std::vector<void *> &dev) {
+  size_t total = 0;
+  for (auto &a : arrs) total += (a.bytes + 255) & ~(size_t)255;
+  char *base = (char *)ctx_stage(c, total + 256);
+  if (!base) return -1;
+  size_t off = 0;
+  dev.clear();
+  for (auto &a : arrs) {
+    dev.push_back(a.host ? base + off : nullptr);
+    if (a.host && a.in) FV3_HIP(hipMemcpyAsync(base + off, a.host, a.bytes, hipMemcpyHostToDevice, c.stream));
+    off += (a.bytes + 255) & ~(size_t)255;
+  }
+  return 0;
+}
+
+static int stage_back(Ctx &c, const std::vector<HostArr> &arrs, const std::vector<void *> &dev) {
+  for (size_t k = 0; k < arrs.size(); k++)
+    if (arrs[k].host && arrs[k].out)
+      FV3_HIP(hipMemcpyAsync(arrs[k].host, dev[k], arrs[k].bytes, hipMemcpyDeviceToHost, c.stream));
+  FV3_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static int check_ctx(fv3_ctx *h, int npx, int npy, int npz) {
+  if (!h) return set_error("NULL context");
+  if (!h->c.grid_set) return set_error("grid metrics not uploaded (fv3_ctx_set_grid)");
+  if (npx != h->c.d.npx || npy != h->c.d.npy) return set_error("npx/npy differ from the context");
+  if (npz > 0 && npz != h->c.npz) return set_error("npz=%d differs from the context (%d)", npz, h->c.npz);
+  return 0;
+}
+
+int fv3_fv_tp_2d(fv3_ctx *h, double *q, const double *crx, const double *cry, int npx, int npy,
+                 int hord, double *fx, double *fy, const double *xfx, const double *yfx,
+                 const double *ra_x, const double *ra_y, double lim_fac, const double *mfx,
+                 const double *mfy, const double *mass, const int *nord, const double *damp_c) {
+  if (check_ctx(h, npx, npy, 0)) return 1;
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (c.ntiles != 1) return set_error("fv3_fv_tp_2d takes a single-tile context");
+  if (!q || !crx || !cry || !fx || !fy || !xfx || !yfx || !ra_x || !ra_y)
+    return set_error("fv3_fv_tp_2d: NULL required argument");
+  if ((mfx == nullptr) != (mfy == nullptr)) return set_error("mfx and mfy must be given together");
+  const Dims &d = c.d;
+  std::vector<HostArr> arrs = {
+      {q, sizeof(double) * d.s2, true, true},          {(void *)crx, sizeof(double) * d.sx, true, false},
+      {(void *)cry, sizeof(double) * d.sy, true, false}, {fx, sizeof(double) * d.smx, false, true},
+      {fy, sizeof(double) * d.smy, false, true},       {(void *)xfx, sizeof(double) * d.sx, true, false},
+      {(void *)yfx, sizeof(double) * d.sy, true, false}, {(void *)ra_x, sizeof(double) * d.srx, true, false},
+      {(void *)ra_y, sizeof(double) * d.sry, true, false}, {(void *)mfx, sizeof(double) * d.smx, true, false},
+      {(void *)mfy, sizeof(double) * d.smy, true, false}, {(void *)mass, sizeof(double) * d.s2, true, false},
+  };
+  std::vector<void *> dv;
+  if (stage_run(c, arrs, dv)) return 1;
+  const bool damp = nord != nullptr && damp_c != nullptr;
+  if (fv_tp_2d_plain<double>(c, (double *)dv[0], (double *)dv[1], (double *)dv[2], hord, (double *)dv[3],
+                             (double *)dv[4], (double *)dv[5], (double *)dv[6], (double *)dv[7],
+                             (double *)dv[8], lim_fac, (double *)dv[9], (double *)dv[10],
+                             (double *)dv[11], damp ? *nord : 0, damp ? *damp_c : 0.0, damp))
+    return 1;
+  return stage_back(c, arrs, dv);
+}
+
+static int tracer_common(fv3_ctx *h, bool per_level, double *q, double *dp1, double *mfx,
+                         double *mfy, double *cx, double *cy, int npx, int npy, int npz, int nq,
+                         int hord, int q_split, double dt, int id_divg, int nord_tr, double trdm,
+                         double lim_fac, int *nsplit_out) {
+  if (check_ctx(h, npx, npy, npz)) return 1;
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!q || !dp1 || !mfx || !mfy || !cx || !cy) return set_error("tracer_2d: NULL array argument");
+  if (nq < 1) return set_error("nq must be >= 1 (got %d)", nq);
+  if (q_split < 0) return set_error("q_split must be >= 0 (got %d)", q_split);
+  const Dims &d = c.d;
+  const long long nt = c.ntiles;
+  std::vector<HostArr> arrs = {
+      {q, sizeof(double) * d.s2 * npz * nq * nt, true, true}, {dp1, sizeof(double) * d.s2 * npz * nt, true, true},
+      {mfx, sizeof(double) * d.smx * npz * nt, true, true},  {mfy, sizeof(double) * d.smy * npz * nt, true, true},
+      {cx, sizeof(double) * d.sx * npz * nt, true, true},    {cy, sizeof(double) * d.sy * npz * nt, true, true},
+  };
+  std::vector<void *> dv;
+  if (stage_run(c, arrs, dv)) return 1;
+  if (tracer_2d_plain<double>(c, per_level, (double *)dv[0], (double *)dv[1], (double *)dv[2], (double *)dv[3],
+                              (double *)dv[4], (double *)dv[5], nq, hord, q_split, dt, id_divg, nord_tr,
+                              trdm, lim_fac, nsplit_out))
+    return 1;
+  return stage_back(c, arrs, dv);
+}
+
+int fv3_tracer_2d_1l(fv3_ctx *h, double *q, double *dp1, double *mfx, double *mfy, double *cx,
+                     double *cy, int npx, int npy, int npz, int nq, int hord, int q_split,
+                     double dt, int id_divg, int nord_tr, double trdm, double lim_fac,
+                     int *nsplit_out) {
+  return tracer_common(h, true, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, hord, q_split, dt,
+                       id_divg, nord_tr, trdm, lim_fac, nsplit_out);
+}
+
+int fv3_tracer_2d(fv3_ctx *h, double *q, double *dp1, double *mfx, double *mfy, double *cx,
+                  double *cy, int npx, int npy, int npz, int nq, int hord, int q_split, double dt,
+                  int id_divg, int nord_tr, double trdm, double lim_fac, int *nsplit_out) {
+  return tracer_common(h, false, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, hord, q_split, dt,
+                       id_divg, nord_tr, trdm, lim_fac, nsplit_out);
+}
+
+// ---- row entry points (stateless) ---------------------------------------------
+static bool kord_ok(int kord) { return kord > 7 && (kord <= 10 || kord > 16); }
+
+static int row_map(int variant, int km, const double *pe1, const double *q1, const double *qs,
+                   int kn, const double *pe2, double *q2, const double *dp2, int i1, int i2,
+                   int j, int ibeg, int iend, int jbeg, int jend, int iv, int kord, bool use_qmin,
+                   double qmin) {
+  if (!pe1 || !q1 || !pe2 || !q2) return set_error("map: NULL array argument");
+  if (km < 4 || kn < 1) return set_error("map: km >= 4 and kn >= 1 required");
+  if (!kord_ok(kord)) return set_error("kord=%d: implemented orders are 8, 9, 10 and > 16", kord);
+  if (i1 > i2 || i1 < ibeg || i2 > iend || j < jbeg || j > jend) return set_error("map: bounds out of range");
+  if (iv == -2 && !qs) return set_error("map: iv=-2 needs qs");
+  Ctx &c = *row_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const long long im = i2 - i1 + 1, ni = iend - ibeg + 1, plane = ni * (jend - jbeg + 1);
+  // gather the row's columns contiguously
+  std::vector<double> q1r(im * km), pe1h(pe1, pe1 + im * (km + 1)), pe2h(pe2, pe2 + im * (kn + 1));
+  for (int k = 0; k < km; k++)
+    for (long long i = 0; i < im; i++) q1r[k * im + i] = q1[k * plane + (long long)(j - jbeg) * ni + (i1 - ibeg) + i];
+  std::vector<HostArr> arrs = {
+      {pe1h.data(), sizeof(double) * im * (km + 1), true, false},
+      {pe2h.data(), sizeof(double) * im * (kn + 1), true, false},
+      {q1r.data(), sizeof(double) * im * km, true, false},
+      {(void *)qs, sizeof(double) * im, true, false},
+      {(void *)dp2, sizeof(double) * im * kn, true, false},
+  };
+  std::vector<double> q2r(im * kn);
+  arrs.push_back({q2r.data(), sizeof(double) * im * kn, false, true});
+  std::vector<void *> dv;
+  if (stage_run(c, arrs, dv)) return 1;
+  RemapArgs<double> a{};
+  a.mode = 1;
+  a.km = km;
+  a.kn = kn;
+  a.ncol = im;
+  a.pe1_in = (const double *)dv[0];
+  a.pe2_in = (const double *)dv[1];
+  a.q1_in = (const double *)dv[2];
+  a.qs_in = (const double *)dv[3];
+  a.dp2_in = (const double *)dv[4];
+  a.q2_out = (double *)dv[5];
+  a.q1_stride = im;
+  a.q2_stride = im;
+  a.iv = iv;
+  a.kord1 = kord;
+  a.variant = variant;
+  a.use_qmin = use_qmin;
+  a.qmin = qmin;
+  if (remap_launch<double>(c, a)) return 1;
+  if (stage_back(c, arrs, dv)) return 1;
+  for (int k = 0; k < kn; k++)
+    for (long long i = 0; i < im; i++) q2[k * plane + (long long)(j - jbeg) * ni + (i1 - ibeg) + i] = q2r[k * im + i];
+  return 0;
+}
+
+int fv3_map1_ppm(int km, const double *pe1, const double *q1, const double *qs, int kn,
+                 const double *pe2, double *q2, int i1, int i2, int j, int ibeg, int iend,
+                 int jbeg, int jend, int iv, int kord) {
+  return row_map(0, km, pe1, q1, qs, kn, pe2, q2, nullptr, i1, i2, j, ibeg, iend, jbeg, jend, iv,
+                 kord, false, 0.0);
+}
+
+int fv3_map_scalar(int km, const double *pe1, const double *q1, const double *qs, int kn,
+                   const double *pe2, double *q2, int i1, int i2, int j, int ibeg, int iend,
+                   int jbeg, int jend, int iv, int kord, double q_min) {
+  return row_map(1, km, pe1, q1, qs, kn, pe2, q2, nullptr, i1, i2, j, ibeg, iend, jbeg, jend, iv,
+                 kord, true, q_min);
+}
+
+int fv3_mapn_tracer(int nq, int km, const double *pe1, const double *pe2, double *q1,
+                    const double *dp2, const int *kord, int j, int i1, int i2, int isd, int ied,
+                    int jsd, int jed, double q_min, int fill) {
+  if (!pe1 || !pe2 || !q1 || !dp2 || !kord) return set_error("mapn_tracer: NULL array argument");
+  if (nq < 1 || nq > REMAP_MAX_NQ) return set_error("mapn_tracer: nq=%d out of 1..%d", nq, REMAP_MAX_NQ);
+  if (km < 4) return set_error("mapn_tracer: km >= 4 required");
+  for (int iq = 0; iq < nq; iq++)
+    if (!kord_ok(kord[iq])) return set_error("kord(%d)=%d: implemented orders are 8, 9, 10 and > 16", iq + 1, kord[iq]);
+  if (i1 > i2 || i1 < isd || i2 > ied || j < jsd || j > jed) return set_error("mapn_tracer: bounds out of range");
+  Ctx &c = *row_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const long long im = i2 - i1 + 1, ni = ied - isd + 1, plane = ni * (jed - jsd + 1);
+  std::vector<double> qr(im * km * nq), pe1h(pe1, pe1 + im * (km + 1)), pe2h(pe2, pe2 + im * (km + 1));
+  for (int iq = 0; iq < nq; iq++)
+    for (int k = 0; k < km; k++)
+      for (long long i = 0; i < im; i++)
+        qr[((long long)iq * km + k) * im + i] = q1[((long long)iq * km + k) * plane + (long long)(j - jsd) * ni + (i1 - isd) + i];
+  std::vector<double> dp2h(dp2, dp2 + im * km);
+  std::vector<HostArr> arrs = {
+      {pe1h.data(), sizeof(double) * im * (km + 1), true, false},
+      {pe2h.data(), sizeof(double) * im * (km + 1), true, false},
+      {qr.data(), sizeof(double) * im * km * nq, true, true},
+      {dp2h.data(), sizeof(double) * im * km, true, false},
+  };
+  std::vector<void *> dv;
+  if (stage_run(c, arrs, dv)) return 1;
+  // one tracer at a time through the mapn_tracer arithmetic (mode 1, variant 3)
+  for (int iq = 0; iq < nq; iq++) {
+    RemapArgs<double> a{};
+    a.mode = 1;
+    a.km = km;
+    a.kn = km;
+    a.ncol = im;
+    a.pe1_in = (const double *)dv[0];
+    a.pe2_in = (const double *)dv[1];
+    a.q1_in = (const double *)dv[2] + (long long)iq * km * im;
+    a.q2_out = (double *)dv[2] + (long long)iq * km * im;
+    a.dp2_in = (const double *)dv[3];
+    a.q1_stride = im;
+    a.q2_stride = im;
+    a.iv = 0;
+    a.kord1 = kord[iq];
+    a.variant = 3;
+    a.use_qmin = true;
+    a.qmin = q_min;
+    a.fill = fill;
+    if (remap_launch<double>(c, a)) return 1;
+  }
+  if (stage_back(c, arrs, dv)) return 1;
+  for (int iq = 0; iq < nq; iq++)
+    for (int k = 0; k < km; k++)
+      for (long long i = 0; i < im; i++)
+        q1[((long long)iq * km + k) * plane + (long long)(j - jsd) * ni + (i1 - isd) + i] = qr[((long long)iq * km + k) * im + i];
+  return 0;
+}
+
+int fv3_cs_profile(const double *qs, double *a4, const double *delp, int km, int i1, int i2,
+                   int iv, int kord) {
+  if (!a4 || !delp) return set_error("cs_profile: NULL array argument");
+  if (km < 4 || i1 > i2) return set_error("cs_profile: km >= 4 and i1 <= i2 required");
+  if (!(kord_ok(kord) || (kord >= 1 && kord < 9))) return set_error("kord=%d not implemented", kord);
+  if (iv == -2 && !qs) return set_error("cs_profile: iv=-2 needs qs");
+  Ctx &c = *row_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const long long im = i2 - i1 + 1;
+  std::vector<double> a1(im * km);
+  for (int k = 0; k < km; k++)
+    for (long long i = 0; i < im; i++) a1[k * im + i] = a4[((long long)k * im + i) * 4 + 0];
+  std::vector<double> dph(delp, delp + im * km);
+  std::vector<double> out(4 * im * km);
+  std::vector<HostArr> arrs = {
+      {a1.data(), sizeof(double) * im * km, true, false},
+      {dph.data(), sizeof(double) * im * km, true, false},
+      {(void *)qs, sizeof(double) * im, true, false},
+      {out.data(), sizeof(double) * 4 * im * km, false, true},
+  };
+  std::vector<void *> dv;
+  if (stage_run(c, arrs, dv)) return 1;
+  if (cs_profile_rows(c, (const double *)dv[0], (const double *)dv[1], (const double *)dv[2],
+                      (double *)dv[3], (int)im, km, iv, kord))
+    return 1;
+  if (stage_back(c, arrs, dv)) return 1;
+  for (int k = 0; k < km; k++)
+    for (long long i = 0; i < im; i++)
+      for (int m = 0; m < 4; m++) a4[((long long)k * im + i) * 4 + m] = out[((long long)m * km + k) * im + i];
+  return 0;
+}
+
+int fv3_lagrangian_to_eulerian_tracers(fv3_ctx *h, int km, int nq, const double *ak,
+                                       const double *bk, double ptop, double *delp, double *q,
+                                       double *pt, const int *kord_tr, int kord_tm, double t_min,
+                                       int fill) {
+  if (!h) return set_error("NULL context");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (km != c.npz) return set_error("km=%d differs from the context npz=%d", km, c.npz);
+  if (!ak || !bk || !delp || (nq > 0 && (!q || !kord_tr))) return set_error("remap: NULL array argument");
+  const Dims &d = c.d;
+  const long long nt = c.ntiles;
+  std::vector<HostArr> arrs = {
+      {(void *)ak, sizeof(double) * (km + 1), true, false},
+      {(void *)bk, sizeof(double) * (km + 1), true, false},
+      {delp, sizeof(double) * d.s2 * km * nt, true, true},
+      {q, sizeof(double) * d.s2 * km * nq * nt, true, true},
+      {pt, sizeof(double) * d.s2 * km * nt, true, true},
+  };
+  std::vector<void *> dv;
+  if (stage_run(c, arrs, dv)) return 1;
+  if (remap_tracers_plain<double>(c, km, nq, (const double *)dv[0], (const double *)dv[1], ptop,
+                                  (double *)dv[2], (double *)dv[3], (double *)dv[4], kord_tr,
+                                  kord_tm, t_min, fill))
+    return 1;
+  return stage_back(c, arrs, dv);
+}
+
+// ---- device-resident step -------------------------------------------------------
+int fv3_step_dev(fv3_ctx *h, int form, int nq, const void *q_in, void *q_out, void *pt,
+                 const double *dp1, double *delp_out, const void *cx, const void *cy,
+                 const void *mfx, const void *mfy, const double *ak, const double *bk,
+                 double ptop, int hord, int kord_tr, int kord_tm, double t_min, int nsplit_cap,
+                 void *stream) {
+  if (!h) return set_error("NULL context");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.grid_set) return set_error("grid metrics not uploaded (fv3_ctx_set_grid)");
+  if (!q_in || !q_out || !dp1 || !delp_out || !cx || !cy || !mfx || !mfy || !ak || !bk)
+    return set_error("fv3_step_dev: NULL argument");
+  if (q_in == q_out) return set_error("fv3_step_dev: q_in and q_out must be distinct buffers");
+  if (nq < 1 || nq > REMAP_MAX_NQ) return set_error("nq=%d out of 1..%d", nq, REMAP_MAX_NQ);
+  hipStream_t saved = c.stream;
+  if (stream) c.stream = (hipStream_t)stream;
+  int rc = 0;
+  const Dims &d = c.d;
+  const int npz = c.npz;
+  const long long nt = c.ntiles;
+  std::vector<int> kt(nq, kord_tr);
+  if (form == FV3_FORM_PLAIN64) {
+    // working copies: the plain form runs the in-place Fortran semantics
+    const size_t bq = sizeof(double) * d.s2 * npz * nq * nt, bdp = sizeof(double) * d.s2 * npz * nt;
+    const size_t bcx = sizeof(double) * d.sx * npz * nt, bcy = sizeof(double) * d.sy * npz * nt;
+    const size_t bmx = sizeof(double) * d.smx * npz * nt, bmy = sizeof(double) * d.smy * npz * nt;
+    char *w = (char *)ctx_stage(c, bdp + bcx + bcy + bmx + bmy + 1024);
+    if (!w) {
+      rc = 1;
+    } else {
+      double *wdp = (double *)w, *wcx = (double *)(w + bdp), *wcy = (double *)(w + bdp + bcx);
+      double *wmx = (double *)(w + bdp + bcx + bcy), *wmy = (double *)(w + bdp + bcx + bcy + bmx);
+      hipMemcpyAsync(q_out, q_in, bq, hipMemcpyDeviceToDevice, c.stream);
+      hipMemcpyAsync(wdp, dp1, bdp, hipMemcpyDeviceToDevice, c.stream);
+      hipMemcpyAsync(wcx, cx, bcx, hipMemcpyDeviceToDevice, c.stream);
+      hipMemcpyAsync(wcy, cy, bcy, hipMemcpyDeviceToDevice, c.stream);
+      hipMemcpyAsync(wmx, mfx, bmx, hipMemcpyDeviceToDevice, c.stream);
+      hipMemcpyAsync(wmy, mfy, bmy, hipMemcpyDeviceToDevice, c.stream);
+      hipMemcpyAsync(delp_out, dp1, bdp, hipMemcpyDeviceToDevice, c.stream);
+      rc = tracer_2d_plain<double>(c, true, (double *)q_out, wdp, wmx, wmy, wcx, wcy, nq, hord, 0, 1.0,
+                                   0, 0, 0.0, 1.0, nullptr);
+      if (!rc)
+        rc = remap_tracers_plain<double>(c, npz, nq, ak, bk, ptop, delp_out, (double *)q_out,
+                                         (double *)pt, kt.data(), kord_tm, t_min, 0);
+    }
+  } else if (form == FV3_FORM_FUSED64 || form == FV3_FORM_FUSED32) {
+    rc = fused_step(c, form == FV3_FORM_FUSED32, nq, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx, mfy,
+                    ak, bk, ptop, hord, kord_tr, kord_tm, t_min, nsplit_cap);
+  } else {
+    rc = set_error("unknown form %d", form);
+  }
+  c.stream = saved;
+  return rc;
+}
+
+}  // extern "C"

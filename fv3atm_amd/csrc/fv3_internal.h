@@ -1,0 +1,116 @@
+// fv3_internal.h -- host-side internal interface of libfv3atm_amd (not part
+// of the C-ABI): domain dimensions, device-resident grid metrics, context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fv3_device.h"
+
+namespace fv3 {
+
+// Fortran bounds of one tile and the derived extents of every array shape on
+// the path (fv_grid_bounds_type, moving_nest/fv_moving_nest.F90:177-185).
+struct Dims {
+  int is, ie, js, je, isd, ied, jsd, jed, npx, npy;
+  int nid, njd;  // (isd:ied, jsd:jed)
+  int nix;       // is:ie+1
+  int niy;       // is:ie
+  int nj;        // js:je
+  int njy;       // js:je+1
+  long long s2;  // q / dp1 / metric slab
+  long long sx;  // cx, xfx   (is:ie+1, jsd:jed)
+  long long sy;  // cy, yfx   (isd:ied, js:je+1)
+  long long smx; // mfx       (is:ie+1, js:je)
+  long long smy; // mfy       (is:ie, js:je+1)
+  long long srx; // ra_x      (is:ie, jsd:jed)
+  long long sry; // ra_y      (isd:ied, js:je)
+  long long sdp; // dp2       (is:ie, js:je)
+  unsigned corners;  // bit0 sw, bit1 se, bit2 ne, bit3 nw
+  bool edge;         // !bounded_domain && grid_type < 3
+  bool bounded;
+};
+
+Dims make_dims(int is, int ie, int js, int je, int isd, int ied, int jsd, int jed, int npx,
+               int npy, unsigned corners, bool bounded, int grid_type);
+
+// Grid metrics of all tiles, stacked tile-outermost, in FP64 and FP32.
+template <typename T>
+struct GridDev {
+  const T *area, *rarea, *dx, *dy, *dxa, *dya, *sin_sg, *del6_u, *del6_v;
+  long long s2, sdx, sdy;  // tile strides of (isd:ied,jsd:jed), dx, dy
+};
+
+struct Ctx {
+  Dims d;
+  int npz = 0, ntiles = 1;
+  double da_min = 0.0;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  // device metrics
+  double *g64 = nullptr;
+  float *g32 = nullptr;
+  GridDev<double> grid64{};
+  GridDev<float> grid32{};
+  bool grid_set = false;
+  // six-tile edge-halo table (dst, src) flat offsets within a tile-stacked slab set
+  long long *halo_dst = nullptr, *halo_src = nullptr;
+  int halo_n = 0;
+  // scratch arena (grown on demand, reused)
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void *stage = nullptr;  // staging area for host-pointer entry points
+  size_t stage_bytes = 0;
+  std::mutex mu;
+};
+
+// error reporting (thread-local last error string)
+int set_error(const char *fmt, ...);
+#define FV3_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return ::fv3::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+  } while (0)
+
+void *ctx_scratch(Ctx &c, size_t bytes);
+void *ctx_stage(Ctx &c, size_t bytes);
+
+template <typename T>
+const GridDev<T> &grid_of(const Ctx &c);
+
+// ---- plain (staged) transport, FP64 or FP32 --------------------------------
+// All pointers are device pointers with the Fortran layouts of tracer_2d:
+// q (isd:ied,jsd:jed,npz,nq), dp1 (isd:ied,jsd:jed,npz), cx (is:ie+1,jsd:jed,npz),
+// cy (isd:ied,js:je+1,npz), mfx (is:ie+1,js:je,npz), mfy (is:ie,js:je+1,npz),
+// each repeated per tile (tile outermost).  In-place semantics as upstream.
+template <typename T>
+int tracer_2d_plain(Ctx &c, bool per_level, T *q, T *dp1, T *mfx, T *mfy, T *cx, T *cy, int nq,
+                    int hord, int q_split, T dt, int id_divg, int nord_tr, T trdm, T lim_fac,
+                    int *nsplit_host);
+
+template <typename T>
+int fv_tp_2d_plain(Ctx &c, T *q, const T *crx, const T *cry, int hord, T *fx, T *fy,
+                   const T *xfx, const T *yfx, const T *ra_x, const T *ra_y, T lim_fac,
+                   const T *mfx, const T *mfy, const T *mass, int nord, T damp_c, bool damp);
+
+// six-tile edge-halo refresh of nslab stacked slabs per tile
+template <typename T>
+int halo_fill(Ctx &c, T *base, long long tile_stride, const int *slab_ids, int nslab_list,
+              int nslab_dense);
+
+// ---- remap ------------------------------------------------------------------
+template <typename T>
+int remap_tracers_plain(Ctx &c, int km, int nq, const double *ak, const double *bk, double ptop,
+                        double *delp, T *q, T *pt, const int *kord_tr, int kord_tm, double t_min,
+                        int fill);
+
+// cs_profile over im independent columns: a1 (km, im) in, out (4, km, im)
+int cs_profile_rows(Ctx &c, const double *a1, const double *dp, const double *qs, double *out,
+                    int im, int km, int iv, int kord);
+
+}  // namespace fv3

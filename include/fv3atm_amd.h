@@ -1,0 +1,135 @@
+/*
+ * fv3atm_amd.h -- C-ABI of the MI355X tracer transport + vertical remap path
+ * (libfv3atm_amd.so).  Plain pointers and sizes only; every function returns
+ * 0 on success and a nonzero status on failure, with the reason in
+ * fv3_last_error().  Host arrays use the dycore's Fortran layout: column-major,
+ * i fastest, explicit lower bounds (isd:ied, jsd:jed, 1:npz, 1:nq); a context
+ * created with ntiles = 6 takes six such arrays stacked tile-outermost.
+ *
+ * The interfaces replaced live in the GFDL_atmos_cubed_sphere submodule, which
+ * is empty in the reference (.gitmodules:1-4); they are reached from
+ * atmosphere_dynamics (atmos_model.F90:879) via fv_dynamics, and fv_mapz_mod is
+ * imported at moving_nest/fv_moving_nest.F90:85.  Argument order and meaning
+ * follow the upstream Fortran subroutines named in each comment.
+ */
+#ifndef FV3ATM_AMD_H
+#define FV3ATM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fv3_ctx fv3_ctx;
+
+/* fv_grid_bounds_type (moving_nest/fv_moving_nest.F90:177-185) */
+typedef struct {
+  int is, ie, js, je, isd, ied, jsd, jed;
+} fv3_bounds;
+
+/* The fv_grid_type components the path reads (moving_nest/fv_moving_nest.F90:81).
+ * Host FP64 arrays of one tile: area, rarea, dxa, dya (isd:ied, jsd:jed);
+ * dx, del6_u (isd:ied, jsd:jed+1); dy, del6_v (isd:ied+1, jsd:jed);
+ * sin_sg (isd:ied, jsd:jed, 9). */
+typedef struct {
+  const double *area, *rarea, *dx, *dy, *dxa, *dya, *sin_sg, *del6_u, *del6_v;
+  double da_min;
+  int sw_corner, se_corner, nw_corner, ne_corner, bounded_domain, grid_type;
+} fv3_grid;
+
+/* Precision / form selectors of the device-resident step. */
+enum {
+  FV3_FORM_PLAIN64 = 0, /* staged kernels, FP64 everywhere (parity form) */
+  FV3_FORM_FUSED64 = 1, /* fused transport kernel, FP64 */
+  FV3_FORM_FUSED32 = 2  /* fused transport kernel, FP32 bulk, FP64 pressure/thickness */
+};
+
+/* Thread-local text of the last failure. */
+const char *fv3_last_error(void);
+
+/* Context: domain bounds, levels, tile count (1 or 6), the device-resident
+ * grid metrics and scratch.  Replaces the state fv_dynamics keeps in
+ * gridstruct/bd/domain (the handles tracer_2d receives). */
+int fv3_ctx_create(const fv3_bounds *bd, int npx, int npy, int npz, int ntiles, fv3_ctx **out);
+int fv3_ctx_destroy(fv3_ctx *ctx);
+/* Upload the metrics of tile (1-based). */
+int fv3_ctx_set_grid(fv3_ctx *ctx, int tile, const fv3_grid *g);
+
+/* tp_core.F90 fv_tp_2d(q, crx, cry, npx, npy, hord, fx, fy, xfx, yfx,
+ *   gridstruct, bd, ra_x, ra_y, lim_fac [, mfx, mfy, mass, nord, damp_c]).
+ * Single-tile context.  Optional arguments are NULL.  q's corner halos are
+ * modified as upstream. */
+int fv3_fv_tp_2d(fv3_ctx *ctx, double *q, const double *crx, const double *cry, int npx,
+                 int npy, int hord, double *fx, double *fy, const double *xfx,
+                 const double *yfx, const double *ra_x, const double *ra_y, double lim_fac,
+                 const double *mfx, const double *mfy, const double *mass, const int *nord,
+                 const double *damp_c);
+
+/* fv_tracer2d.F90 tracer_2d_1L(q, dp1, mfx, mfy, cx, cy, gridstruct, bd,
+ *   domain, npx, npy, npz, nq, hord, q_split, dt, id_divg, q_pack, dp1_pack,
+ *   nord_tr, trdm, lim_fac): per-level sub-cycling, plain FP64 form.
+ * All arrays inout as upstream (cx, cy, mfx, mfy scaled by 1/nsplit(k)).
+ * nsplit_out (npz, may be NULL) receives the sub-cycle counts. */
+int fv3_tracer_2d_1l(fv3_ctx *ctx, double *q, double *dp1, double *mfx, double *mfy,
+                     double *cx, double *cy, int npx, int npy, int npz, int nq, int hord,
+                     int q_split, double dt, int id_divg, int nord_tr, double trdm,
+                     double lim_fac, int *nsplit_out);
+/* fv_tracer2d.F90 tracer_2d(...): one sub-cycle count for all levels
+ * (q_split > 0 forces it), optional deln_flux on the first sub-cycle. */
+int fv3_tracer_2d(fv3_ctx *ctx, double *q, double *dp1, double *mfx, double *mfy, double *cx,
+                  double *cy, int npx, int npy, int npz, int nq, int hord, int q_split,
+                  double dt, int id_divg, int nord_tr, double trdm, double lim_fac,
+                  int *nsplit_out);
+
+/* fv_mapz.F90 map1_ppm(km, pe1, q1, qs, kn, pe2, q2, i1, i2, j, ibeg, iend,
+ *   jbeg, jend, iv, kord): one row j, columns i1:i2; pe1 (i1:i2, km+1),
+ *   pe2 (i1:i2, kn+1), q1 (ibeg:iend, jbeg:jend, km), q2 (..., kn), qs (i1:i2). */
+int fv3_map1_ppm(int km, const double *pe1, const double *q1, const double *qs, int kn,
+                 const double *pe2, double *q2, int i1, int i2, int j, int ibeg, int iend,
+                 int jbeg, int jend, int iv, int kord);
+/* fv_mapz.F90 map_scalar(..., q_min): as map1_ppm with the scalar_profile
+ * floor q_min. */
+int fv3_map_scalar(int km, const double *pe1, const double *q1, const double *qs, int kn,
+                   const double *pe2, double *q2, int i1, int i2, int j, int ibeg, int iend,
+                   int jbeg, int jend, int iv, int kord, double q_min);
+/* fv_mapz.F90 mapn_tracer(nq, km, pe1, pe2, q1, dp2, kord, j, i1, i2, isd,
+ *   ied, jsd, jed, q_min, fill): all tracers of row j, q1 (isd:ied,jsd:jed,km,nq)
+ *   inout, dp2 (i1:i2, km), kord (nq). */
+int fv3_mapn_tracer(int nq, int km, const double *pe1, const double *pe2, double *q1,
+                    const double *dp2, const int *kord, int j, int i1, int i2, int isd, int ied,
+                    int jsd, int jed, double q_min, int fill);
+/* fv_mapz.F90 cs_profile(qs, a4, delp, km, i1, i2, iv, kord): a4 (4, i1:i2, km)
+ * inout, delp (i1:i2, km), qs (i1:i2). */
+int fv3_cs_profile(const double *qs, double *a4, const double *delp, int km, int i1, int i2,
+                   int iv, int kord);
+
+/* Tracer/pt subset of fv_mapz.F90 Lagrangian_to_Eulerian over every row of
+ * every tile of ctx: pe1 = ptop + cumsum(delp), pe2 = ak + bk*ps, tracers by
+ * mapn_tracer (nq > 5) or map1_q2 (kord_tr, iv = 0, optional fillz), pt by
+ * map_scalar (iv = 1, kord_tm, floor t_min) when pt != NULL, delp <- dp2.
+ * delp, pt (isd:ied, jsd:jed, km); q (isd:ied, jsd:jed, km, nq); ak, bk (km+1). */
+int fv3_lagrangian_to_eulerian_tracers(fv3_ctx *ctx, int km, int nq, const double *ak,
+                                       const double *bk, double ptop, double *delp, double *q,
+                                       double *pt, const int *kord_tr, int kord_tm,
+                                       double t_min, int fill);
+
+/* Device-resident advection + remap step (no counterpart upstream as one call:
+ * it is tracer_2d_1L followed by the tracer/pt subset of
+ * Lagrangian_to_Eulerian, the order fv_dynamics runs them in).  Device
+ * pointers; q_in and q_out are distinct buffers of (isd:ied,jsd:jed,npz,nq)
+ * per tile.  Winds and dp1 are read-only here: the 1/nsplit scaling is
+ * applied in flight.  Element types: form PLAIN64/FUSED64 -> q, pt, cx, cy,
+ * mfx, mfy double; FUSED32 -> float.  dp1 (the Lagrangian thickness) and
+ * delp_out are always double.  pt (may be NULL) is remapped in place; delp_out
+ * receives the target thicknesses.  stream is a hipStream_t (NULL: the
+ * context's stream).  nsplit_cap bounds the per-level sub-cycle count the
+ * fused forms schedule (0 = derive it, with one device-to-host read). */
+int fv3_step_dev(fv3_ctx *ctx, int form, int nq, const void *q_in, void *q_out, void *pt,
+                 const double *dp1, double *delp_out, const void *cx, const void *cy,
+                 const void *mfx, const void *mfy, const double *ak, const double *bk,
+                 double ptop, int hord, int kord_tr, int kord_tm, double t_min,
+                 int nsplit_cap, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,114 @@
+/*
+ * fv3_oracle.h -- CPU FP64 restatement of FV3's tracer transport and vertical
+ * remap path.  TEST INFRASTRUCTURE ONLY: this library is the checker used by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The
+ * product (libfv3atm_amd.so) never links or calls it.
+ *
+ * The upstream sources (GFDL_atmos_cubed_sphere model/tp_core.F90,
+ * model/fv_tracer2d.F90, model/fv_mapz.F90) are an empty submodule in the
+ * reference (.gitmodules:1-4); every routine here is restated from the
+ * published algorithm (Lin & Rood 1996, Lin 2004, Putman & Lin 2007,
+ * Colella & Woodward 1984, Huynh 1996) and from the public upstream interface.
+ * Arrays follow the Fortran conventions of the dycore: column-major, i
+ * fastest, lower bounds carried explicitly (isd:ied, jsd:jed, 1:npz, 1:nq),
+ * as at ccpp/data/CCPP_typedefs.F90:1580-1590 and
+ * moving_nest/fv_moving_nest.F90:177-189.
+ */
+#ifndef FV3_ORACLE_H
+#define FV3_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* fv_grid_bounds_type (moving_nest/fv_moving_nest.F90:177-185). */
+typedef struct {
+  int is, ie, js, je;
+  int isd, ied, jsd, jed;
+} fv3o_bounds;
+
+/* The fv_grid_type components the path reads (moving_nest/fv_moving_nest.F90:81).
+ * Shapes: area, rarea, dxa, dya: (isd:ied, jsd:jed); dx, del6_u: (isd:ied,
+ * jsd:jed+1); dy, del6_v: (isd:ied+1, jsd:jed); sin_sg: (isd:ied, jsd:jed, 9). */
+typedef struct {
+  const double *area, *rarea, *dx, *dy, *dxa, *dya, *sin_sg, *del6_u, *del6_v;
+  double da_min;
+  int sw_corner, se_corner, nw_corner, ne_corner;
+  int bounded_domain, grid_type;
+} fv3o_grid;
+
+/* ---- tp_core ---------------------------------------------------------- */
+void fv3o_copy_corners(double *q, int npx, int npy, int dir, const fv3o_bounds *bd,
+                       const fv3o_grid *g);
+void fv3o_pert_ppm(int im, const double *a0, double *al, double *ar, int iv);
+/* xppm: q(isd:ied, jfirst:jlast), c(is:ie+1, jfirst:jlast), flux(is:ie+1, jfirst:jlast) */
+int fv3o_xppm(double *flux, const double *q, const double *c, int iord, int is, int ie,
+              int isd, int ied, int jfirst, int jlast, int jsd, int jed, int npx, int npy,
+              const double *dxa, int bounded_domain, int grid_type, double lim_fac);
+/* yppm: q(ifirst:ilast, jsd:jed), c(isd:ied, js:je+1), flux(ifirst:ilast, js:je+1) */
+int fv3o_yppm(double *flux, const double *q, const double *c, int jord, int ifirst,
+              int ilast, int isd, int ied, int js, int je, int jsd, int jed, int npx,
+              int npy, const double *dya, int bounded_domain, int grid_type, double lim_fac);
+void fv3o_deln_flux(int nord, int npx, int npy, double damp, const double *q, double *fx,
+                    double *fy, const fv3o_grid *g, const fv3o_bounds *bd, const double *mass);
+/* Optional Fortran arguments are NULL pointers here. */
+int fv3o_fv_tp_2d(double *q, const double *crx, const double *cry, int npx, int npy, int hord,
+                  double *fx, double *fy, const double *xfx, const double *yfx,
+                  const fv3o_grid *g, const fv3o_bounds *bd, const double *ra_x,
+                  const double *ra_y, double lim_fac, const double *mfx, const double *mfy,
+                  const double *mass, const int *nord, const double *damp_c);
+
+/* ---- fv_tracer2d ------------------------------------------------------- */
+/* Halo refresh hook between sub-cycles: called with (ctx, field, nlev-or-nq
+ * stacked 2-D slabs, level k (1-based) or 0 when all levels are refreshed).
+ * NULL keeps halos as given at entry (single tile, frozen-halo policy). */
+typedef void (*fv3o_halo_fn)(void *ctx, double *field, int nslab, int k);
+
+/* Per-level sub-cycling (tracer_2d_1L).  ntiles tiles are stacked outermost
+ * in every array (tile stride = the single-tile array size).  nsplit_out
+ * (npz) receives nsplit(k) when non-NULL. */
+int fv3o_tracer_2d_1L(double *q, double *dp1, double *mfx, double *mfy, double *cx,
+                      double *cy, const fv3o_grid *g, const fv3o_bounds *bd, int ntiles,
+                      int npx, int npy, int npz, int nq, int hord, int q_split, double dt,
+                      int id_divg, int nord_tr, double trdm, double lim_fac,
+                      fv3o_halo_fn halo, void *halo_ctx, int *nsplit_out);
+/* Global sub-cycling (tracer_2d). */
+int fv3o_tracer_2d(double *q, double *dp1, double *mfx, double *mfy, double *cx, double *cy,
+                   const fv3o_grid *g, const fv3o_bounds *bd, int ntiles, int npx, int npy,
+                   int npz, int nq, int hord, int q_split, double dt, int id_divg,
+                   int nord_tr, double trdm, double lim_fac, fv3o_halo_fn halo,
+                   void *halo_ctx, int *nsplit_out);
+
+/* ---- fv_mapz ------------------------------------------------------------ */
+/* a4(4, i1:i2, km), delp(i1:i2, km), qs(i1:i2) */
+void fv3o_cs_limiters(int im, const int *extm, double *a4, int iv);
+int fv3o_cs_profile(const double *qs, double *a4, const double *delp, int km, int i1, int i2,
+                    int iv, int kord);
+int fv3o_scalar_profile(const double *qs, double *a4, const double *delp, int km, int i1,
+                        int i2, int iv, int kord, double qmin);
+int fv3o_map1_ppm(int km, const double *pe1, const double *q1, const double *qs, int kn,
+                  const double *pe2, double *q2, int i1, int i2, int j, int ibeg, int iend,
+                  int jbeg, int jend, int iv, int kord);
+int fv3o_map_scalar(int km, const double *pe1, const double *q1, const double *qs, int kn,
+                    const double *pe2, double *q2, int i1, int i2, int j, int ibeg, int iend,
+                    int jbeg, int jend, int iv, int kord, double q_min);
+int fv3o_map1_q2(int km, const double *pe1, const double *q1, int kn, const double *pe2,
+                 double *q2, const double *dp2, int i1, int i2, int iv, int kord, int j,
+                 int ibeg, int iend, int jbeg, int jend, double q_min);
+int fv3o_mapn_tracer(int nq, int km, const double *pe1, const double *pe2, double *q1,
+                     const double *dp2, const int *kord, int j, int i1, int i2, int isd,
+                     int ied, int jsd, int jed, double q_min, int fill);
+void fv3o_fillz(int im, int km, int nq, double *q, const double *dp);
+/* Tracer/pt subset of Lagrangian_to_Eulerian over all rows js:je.
+ * delp, pt (isd:ied, jsd:jed, km); q (isd:ied, jsd:jed, km, nq); ak, bk (km+1).
+ * pt is remapped with map_scalar (iv=1, kord_tm, t_min) when pt != NULL. */
+int fv3o_lagrangian_to_eulerian_tracers(int km, int nq, const fv3o_bounds *bd,
+                                        const double *ak, const double *bk, double ptop,
+                                        double *delp, double *q, double *pt,
+                                        const int *kord_tr, int kord_tm, double t_min,
+                                        int fill);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

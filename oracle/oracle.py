@@ -1,0 +1,238 @@
+"""ctypes face of the FP64 CPU oracle (oracle/libfv3_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker; the product never imports it.
+The oracle restates the upstream tp_core.F90 / fv_tracer2d.F90 /
+fv_mapz.F90 algorithms (absent submodule, .gitmodules:1-4) in C.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_here = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+NG = 3
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+
+
+class Bounds(ctypes.Structure):
+    _fields_ = [(k, I) for k in ("is_", "ie", "js", "je", "isd", "ied", "jsd", "jed")]
+
+
+class GridC(ctypes.Structure):
+    _fields_ = [(k, P) for k in ("area", "rarea", "dx", "dy", "dxa", "dya", "sin_sg",
+                                 "del6_u", "del6_v")] + [("da_min", D)] + \
+               [(k, I) for k in ("sw_corner", "se_corner", "nw_corner", "ne_corner",
+                                 "bounded_domain", "grid_type")]
+
+
+HALO_FN = ctypes.CFUNCTYPE(None, P, P, I, I)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_here, "libfv3_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run oracle/build_oracle.sh")
+        L = ctypes.CDLL(path)
+        L.fv3o_fv_tp_2d.argtypes = [P, P, P, I, I, I, P, P, P, P, P, P, P, P, D, P, P, P, P, P]
+        L.fv3o_tracer_2d_1L.argtypes = [P, P, P, P, P, P, P, P, I, I, I, I, I, I, I, D, I, I, D,
+                                        D, HALO_FN, P, P]
+        L.fv3o_tracer_2d.argtypes = L.fv3o_tracer_2d_1L.argtypes
+        L.fv3o_copy_corners.argtypes = [P, I, I, I, P, P]
+        L.fv3o_xppm.argtypes = [P, P, P, I, I, I, I, I, I, I, I, I, I, I, P, I, I, D]
+        L.fv3o_yppm.argtypes = [P, P, P, I, I, I, I, I, I, I, I, I, I, I, P, I, I, D]
+        L.fv3o_pert_ppm.argtypes = [I, P, P, P, I]
+        L.fv3o_deln_flux.argtypes = [I, I, I, D, P, P, P, P, P, P]
+        L.fv3o_cs_profile.argtypes = [P, P, P, I, I, I, I, I]
+        L.fv3o_scalar_profile.argtypes = [P, P, P, I, I, I, I, I, D]
+        L.fv3o_map1_ppm.argtypes = [I, P, P, P, I, P, P, I, I, I, I, I, I, I, I, I]
+        L.fv3o_map_scalar.argtypes = [I, P, P, P, I, P, P, I, I, I, I, I, I, I, I, I, D]
+        L.fv3o_mapn_tracer.argtypes = [I, I, P, P, P, P, P, I, I, I, I, I, I, I, D, I]
+        L.fv3o_fillz.argtypes = [I, I, I, P, P]
+        L.fv3o_lagrangian_to_eulerian_tracers.argtypes = [I, I, P, P, P, D, P, P, P, P, I, D, I]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+def bounds(n: int) -> Bounds:
+    return Bounds(1, n, 1, n, 1 - NG, n + NG, 1 - NG, n + NG)
+
+
+def grid_struct(g) -> GridC:
+    return GridC(_p(g.area), _p(g.rarea), _p(g.dx), _p(g.dy), _p(g.dxa), _p(g.dya),
+                 _p(g.sin_sg), _p(g.del6_u), _p(g.del6_v), g.da_min, int(g.sw_corner),
+                 int(g.se_corner), int(g.nw_corner), int(g.ne_corner), int(g.bounded_domain),
+                 g.grid_type)
+
+
+def _f(a):
+    assert a.dtype == np.float64 and a.flags.f_contiguous
+    return a
+
+
+class SphereHalo:
+    """Edge-halo refresh of six stacked tiles by the cube topology (the
+    mpp_update_domains equivalent on one process)."""
+
+    def __init__(self, n: int):
+        from fv3atm_amd import cases
+        self.n = n
+        nd = n + 2 * NG
+        self.s2 = nd * nd
+        maps = [cases.halo_map(n, t).ravel(order="F") for t in range(1, 7)]
+        self.dst = [np.nonzero(m >= 0)[0] for m in maps]
+        self.src = [m[m >= 0] for m in maps]
+
+    def fill(self, field: np.ndarray, nslab: int, per_tile_slabs: int, k: int, slab_stride=None):
+        """field: flat float64 view; slabs of s2 per tile; tiles outermost."""
+        s2 = self.s2
+        for s in range(nslab):
+            off = s * s2
+            stack = np.stack([field[t * per_tile_slabs * s2 + off: t * per_tile_slabs * s2 + off + s2]
+                              for t in range(6)]).ravel()
+            for t in range(6):
+                base = t * per_tile_slabs * s2 + off
+                field[base + self.dst[t]] = stack[self.src[t]]
+
+
+def fv_tp_2d(q, crx, cry, hord, xfx, yfx, g, ra_x, ra_y, lim_fac=1.0, mfx=None, mfy=None,
+             mass=None, nord=None, damp_c=None):
+    """Returns (fx, fy); q's corner halos are modified in place as upstream."""
+    n = g.n
+    fx = np.zeros((n + 1, n), order="F")
+    fy = np.zeros((n, n + 1), order="F")
+    bd = bounds(n)
+    gs = grid_struct(g)
+    nord_c = ctypes.c_int(nord) if nord is not None else None
+    damp_cc = ctypes.c_double(damp_c) if damp_c is not None else None
+    rc = lib().fv3o_fv_tp_2d(_p(_f(q)), _p(_f(crx)), _p(_f(cry)), g.npx, g.npy, hord, _p(fx),
+                             _p(fy), _p(_f(xfx)), _p(_f(yfx)), ctypes.byref(gs),
+                             ctypes.byref(bd), _p(_f(ra_x)), _p(_f(ra_y)), lim_fac, _p(mfx),
+                             _p(mfy), _p(mass),
+                             ctypes.byref(nord_c) if nord_c is not None else None,
+                             ctypes.byref(damp_cc) if damp_cc is not None else None)
+    if rc:
+        raise ValueError("fv3o_fv_tp_2d failed")
+    return fx, fy
+
+
+def tracer_2d(case, variant="1L", q=None, dp1=None, cx=None, cy=None, mfx=None, mfy=None,
+              nord_tr=0, trdm=0.0, id_divg=0):
+    """Runs the oracle tracer_2d[_1L] on copies (or the given arrays, in place).
+    Returns dict(q, dp1, cx, cy, mfx, mfy, nsplit)."""
+    c = case
+    arr = {k: np.array(v, order="F", copy=True) if v is None or True else v
+           for k, v in dict(q=c.q if q is None else q, dp1=c.dp1 if dp1 is None else dp1,
+                            cx=c.cx if cx is None else cx, cy=c.cy if cy is None else cy,
+                            mfx=c.mfx if mfx is None else mfx,
+                            mfy=c.mfy if mfy is None else mfy).items()}
+    grids = (GridC * c.ntiles)(*[grid_struct(g) for g in c.grids])
+    bd = bounds(c.n)
+    nsplit = np.zeros(c.npz, dtype=np.int32)
+    halo_cb = HALO_FN()
+    keep = None
+    if c.ntiles == 6:
+        sh = SphereHalo(c.n)
+        qflat = arr["q"].reshape(-1, order="F")
+
+        def _halo(ctx, ptr, nslab, k):
+            buf = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)),
+                                        shape=(6 * (c.npz * c.nq if k == 0 else nslab) * sh.s2,))
+            per = c.npz * c.nq if k == 0 else nslab
+            sh.fill(buf, nslab, per, k)
+
+        keep = HALO_FN(_halo)
+        halo_cb = keep
+    fn = lib().fv3o_tracer_2d_1L if variant == "1L" else lib().fv3o_tracer_2d
+    rc = fn(_p(arr["q"]), _p(arr["dp1"]), _p(arr["mfx"]), _p(arr["mfy"]), _p(arr["cx"]),
+            _p(arr["cy"]), grids, ctypes.byref(bd), c.ntiles, c.n + 1, c.n + 1, c.npz, c.nq,
+            c.hord, c.q_split, c.dt, id_divg, nord_tr, trdm, c.lim_fac, halo_cb, None,
+            _p(nsplit))
+    if rc:
+        raise ValueError("oracle tracer_2d failed")
+    arr["nsplit"] = nsplit
+    return arr
+
+
+def remap_tracers(case, q, delp, pt=None, kord_tr=None, kord_tm=None, t_min=184.0, fill=False):
+    """Tracer/pt subset of Lagrangian_to_Eulerian on copies; returns (q, delp, pt)."""
+    c = case
+    q = np.array(q, order="F", copy=True)
+    delp = np.array(delp, order="F", copy=True)
+    pt = None if pt is None else np.array(pt, order="F", copy=True)
+    kt = np.full(c.nq, c.kord_tr if kord_tr is None else kord_tr, dtype=np.int32)
+    bd = bounds(c.n)
+    ntiles = c.ntiles
+    for t in range(ntiles):
+        qt = q[..., t] if ntiles > 1 else q
+        dt_ = delp[..., t] if ntiles > 1 else delp
+        ptt = None if pt is None else (pt[..., t] if ntiles > 1 else pt)
+        qt_c = np.asfortranarray(qt)
+        dt_c = np.asfortranarray(dt_)
+        pt_c = None if ptt is None else np.asfortranarray(ptt)
+        rc = lib().fv3o_lagrangian_to_eulerian_tracers(
+            c.npz, c.nq, ctypes.byref(bd), _p(c.ak), _p(c.bk), c.ptop, _p(dt_c), _p(qt_c),
+            _p(pt_c), _p(kt), c.kord_tm if kord_tm is None else kord_tm, t_min, int(fill))
+        if rc:
+            raise ValueError("oracle remap failed")
+        if ntiles > 1:
+            q[..., t] = qt_c
+            delp[..., t] = dt_c
+            if pt is not None:
+                pt[..., t] = pt_c
+        else:
+            q, delp, pt = qt_c, dt_c, pt_c
+    return q, delp, pt
+
+
+def cs_profile(a1, delp, iv=0, kord=9, qs=None, qmin=None):
+    """a1, delp: (ncol, km) arrays.  Returns a4 (4, ncol, km) Fortran order."""
+    ncol, km = a1.shape
+    a4 = np.zeros((4, ncol, km), order="F")
+    a4[0] = a1
+    dp = np.asfortranarray(delp, dtype=np.float64)
+    qsa = np.zeros(ncol) if qs is None else np.ascontiguousarray(qs, dtype=np.float64)
+    if qmin is None:
+        rc = lib().fv3o_cs_profile(_p(qsa), _p(a4), _p(dp), km, 1, ncol, iv, kord)
+    else:
+        rc = lib().fv3o_scalar_profile(_p(qsa), _p(a4), _p(dp), km, 1, ncol, iv, kord, qmin)
+    if rc:
+        raise ValueError("cs_profile failed")
+    return a4
+
+
+def map1_ppm(pe1, q1, pe2, iv=0, kord=9, qs=None):
+    """Column form: pe1 (ncol, km+1), q1 (ncol, km), pe2 (ncol, kn+1) -> q2 (ncol, kn)."""
+    ncol, km = q1.shape
+    kn = pe2.shape[1] - 1
+    pe1f = np.asfortranarray(pe1, dtype=np.float64)
+    pe2f = np.asfortranarray(pe2, dtype=np.float64)
+    q1f = np.asfortranarray(q1, dtype=np.float64).reshape(ncol, 1, km, order="F")
+    q2f = np.zeros((ncol, 1, kn), order="F")
+    qsa = np.zeros(ncol) if qs is None else np.ascontiguousarray(qs, dtype=np.float64)
+    rc = lib().fv3o_map1_ppm(km, _p(pe1f), _p(q1f), _p(qsa), kn, _p(pe2f), _p(q2f), 1, ncol, 1,
+                             1, ncol, 1, 1, iv, kord)
+    if rc:
+        raise ValueError("map1_ppm failed")
+    return q2f.reshape(ncol, kn, order="F")
+
+
+def fillz(q, dp):
+    """q (im, km, nq), dp (im, km); in place on a copy."""
+    q = np.array(q, order="F", copy=True, dtype=np.float64)
+    dp = np.asfortranarray(dp, dtype=np.float64)
+    im, km, nq = q.shape
+    lib().fv3o_fillz(im, km, nq, _p(q), _p(dp))
+    return q

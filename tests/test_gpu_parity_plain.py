@@ -1,0 +1,183 @@
+"""GPU parity of the plain FP64 form against the FP64 CPU oracle.
+
+Every entry point is called through the C-ABI (fv3atm_amd mirror of the
+Fortran interfaces) on the same seeded synthetic inputs the oracle gets; the
+bar is 1e-12 relative per cell on every inout array (the FP64 form is
+expected to reproduce the oracle bit for bit).
+"""
+import numpy as np
+import pytest
+
+import fv3atm_amd as fa
+from fv3atm_amd import cases
+from oracle import oracle
+from tests._util import assert_close, interior
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-12
+
+
+def _copy(c):
+    return {k: np.array(getattr(c, k), order="F", copy=True)
+            for k in ("q", "dp1", "cx", "cy", "mfx", "mfy")}
+
+
+def _run_tracer(case, variant, nord_tr=0, trdm=0.0, id_divg=0):
+    ctx = fa.Context(case.grids, case.npz)
+    a = _copy(case)
+    fn = fa.tracer_2d_1L if variant == "1L" else fa.tracer_2d
+    nsplit = fn(ctx, a["q"], a["dp1"], a["mfx"], a["mfy"], a["cx"], a["cy"], case.n + 1,
+                case.n + 1, case.npz, case.nq, case.hord, case.q_split, case.dt, id_divg,
+                nord_tr, trdm, case.lim_fac)
+    ctx.close()
+    ref = oracle.tracer_2d(case, variant, nord_tr=nord_tr, trdm=trdm, id_divg=id_divg)
+    return a, nsplit, ref
+
+
+def _check_all(a, nsplit, ref, label):
+    assert np.array_equal(nsplit, ref["nsplit"]), f"{label}: nsplit differs"
+    for k in ("q", "dp1", "cx", "cy", "mfx", "mfy"):
+        assert_close(a[k], ref[k], RTOL, f"{label} {k}")
+
+
+@pytest.mark.parametrize("hord", [8, 10, 13])
+def test_tracer_2d_1L_c48(gpu_ready, hord):
+    case = cases.config(0)
+    case.hord = hord
+    a, ns, ref = _run_tracer(case, "1L")
+    print("C48L32Q1 hord", hord, "nsplit", np.unique(ns))
+    _check_all(a, ns, ref, f"C48 hord{hord}")
+
+
+def test_tracer_2d_1L_subcycled(gpu_ready):
+    # jet profile: uneven nsplit(k) in 1..3, frozen single-tile halos between sub-cycles
+    case = cases.make_case("C24L12Q3jet", n=24, npz=12, nq=3, jet=True, courant=2.5,
+                           delp_kind=1, delp_amp=(0.05, 0.2), seed=3)
+    a, ns, ref = _run_tracer(case, "1L")
+    print("nsplit", ns)
+    assert ns.max() >= 2 and ns.min() == 1
+    _check_all(a, ns, ref, "C24 jet")
+
+
+@pytest.mark.parametrize("nord,trdm", [(0, 0.0), (0, 0.12), (1, 0.1), (2, 0.08)])
+def test_tracer_2d_forced_split_damping(gpu_ready, nord, trdm):
+    case = cases.make_case("C16L6Q2", n=16, npz=6, nq=2, wind="deform", courant=0.7,
+                           q_split=2, delp_kind=1, delp_amp=(0.05, 0.1), seed=5)
+    a, ns, ref = _run_tracer(case, "2d", nord_tr=nord, trdm=trdm)
+    assert (ns == 2).all()
+    _check_all(a, ns, ref, f"tracer_2d nord{nord} trdm{trdm}")
+
+
+def test_tracer_2d_divg(gpu_ready):
+    case = cases.make_case("C12L4Q1", n=12, npz=4, nq=1, courant=1.4, seed=9)
+    a, ns, ref = _run_tracer(case, "2d", id_divg=1)
+    _check_all(a, ns, ref, "tracer_2d id_divg")
+
+
+@pytest.mark.parametrize("with_mf,damp", [(True, None), (False, None), (True, (1, 0.15)),
+                                          (False, (2, 0.1))])
+def test_fv_tp_2d(gpu_ready, with_mf, damp):
+    case = cases.make_case("C20L1Q1", n=20, npz=1, nq=1, courant=0.8, seed=2)
+    g = case.grids[0]
+    n = case.n
+    q = np.array(case.q[:, :, 0, 0], order="F")
+    cx, cy = np.asfortranarray(case.cx[..., 0]), np.asfortranarray(case.cy[..., 0])
+    # area fluxes and ra_x/ra_y as tracer_2d forms them
+    xfx = np.zeros_like(cx)
+    yfx = np.zeros_like(cy)
+    ng = 3
+    for j in range(n + 6):
+        for ii in range(n + 1):
+            i = ii + ng  # index into (isd:ied) arrays of face i=ii+1
+            c = cx[ii, j]
+            if c > 0:
+                xfx[ii, j] = c * g.dxa[i - 1, j] * g.dy[i, j] * g.sin_sg[i - 1, j, 2]
+            else:
+                xfx[ii, j] = c * g.dxa[i, j] * g.dy[i, j] * g.sin_sg[i, j, 0]
+    for j in range(n + 1):
+        for i in range(n + 6):
+            c = cy[i, j]
+            jj = j + ng
+            if c > 0:
+                yfx[i, j] = c * g.dya[i, jj - 1] * g.dx[i, jj] * g.sin_sg[i, jj - 1, 3]
+            else:
+                yfx[i, j] = c * g.dya[i, jj] * g.dx[i, jj] * g.sin_sg[i, jj, 1]
+    ra_x = np.asfortranarray(g.area[ng:n + ng, :] + xfx[:-1, :] - xfx[1:, :])
+    ra_y = np.asfortranarray(g.area[:, ng:n + ng] + yfx[:, :-1] - yfx[:, 1:])
+    mfx = np.asfortranarray(case.mfx[..., 0]) if with_mf else None
+    mfy = np.asfortranarray(case.mfy[..., 0]) if with_mf else None
+    mass = np.asfortranarray(case.dp1[..., 0]) if with_mf else None
+    nord, damp_c = (damp if damp else (None, None))
+    qg = q.copy(order="F")
+    fx = np.zeros((n + 1, n), order="F")
+    fy = np.zeros((n, n + 1), order="F")
+    ctx = fa.Context(case.grids, 1)
+    fa.fv_tp_2d(ctx, qg, cx, cy, n + 1, n + 1, 8, fx, fy, xfx, yfx, ra_x, ra_y, 1.0, mfx, mfy,
+                mass, nord, damp_c)
+    ctx.close()
+    qo = q.copy(order="F")
+    rfx, rfy = oracle.fv_tp_2d(qo, cx, cy, 8, xfx, yfx, g, ra_x, ra_y, 1.0, mfx, mfy, mass,
+                               nord, damp_c)
+    assert_close(fx, rfx, RTOL, "fx", floor=1e-300)
+    assert_close(fy, rfy, RTOL, "fy", floor=1e-300)
+    assert_close(qg, qo, RTOL, "q corners")
+
+
+def _remap_case(nq, seed, fill=False):
+    case = cases.make_case(f"C12L16Q{nq}", n=12, npz=16, nq=nq, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=seed, with_pt=True)
+    q = np.array(case.q, order="F")
+    if fill:  # a few negative values so fillz has work
+        q[5, 6, 3, 0] = -0.3
+        q[7, 7, 0, 0] = -0.2
+    return case, q
+
+
+@pytest.mark.parametrize("nq,kord_tr,kord_tm,fill", [(3, 9, 10, False), (7, 9, 10, True),
+                                                     (2, 10, 9, True), (6, 8, 17, False)])
+def test_remap_tracers(gpu_ready, nq, kord_tr, kord_tm, fill):
+    case, q = _remap_case(nq, seed=nq, fill=fill)
+    delp = np.array(case.dp1, order="F")
+    pt = np.array(case.pt, order="F")
+    ctx = fa.Context(case.grids, case.npz)
+    qg, dg, pg = q.copy(order="F"), delp.copy(order="F"), pt.copy(order="F")
+    fa.lagrangian_to_eulerian_tracers(ctx, case.npz, nq, case.ak, case.bk, case.ptop, dg, qg, pg,
+                                      kord_tr, kord_tm, 184.0, fill)
+    ctx.close()
+    rq, rd, rp = oracle.remap_tracers(case, q, delp, pt, kord_tr=kord_tr, kord_tm=kord_tm,
+                                      fill=fill)
+    assert_close(qg, rq, RTOL, "q", floor=1e-300)
+    assert_close(dg, rd, RTOL, "delp")
+    assert_close(pg, rp, RTOL, "pt")
+
+
+def test_row_entries(gpu_ready):
+    rng = np.random.default_rng(4)
+    im, km, kn = 9, 12, 10
+    dp1 = rng.uniform(0.5, 1.5, (im, km))
+    pe1 = np.concatenate([np.zeros((im, 1)), np.cumsum(dp1, axis=1)], axis=1)
+    pe2 = np.zeros((im, kn + 1))
+    for i in range(im):
+        pe2[i] = np.concatenate([[0.0], np.sort(rng.uniform(0, pe1[i, -1], kn - 1)), [pe1[i, -1]]])
+    q1 = rng.uniform(0.0, 2.0, (im, km))
+    pe1f, pe2f = np.asfortranarray(pe1), np.asfortranarray(pe2)
+    for kord in (9, 10):
+        q1f = np.asfortranarray(q1.reshape(im, 1, km))
+        q2 = np.zeros((im, 1, kn), order="F")
+        fa.map1_ppm(km, pe1f, q1f, np.zeros(im), kn, pe2f, q2, 1, im, 1, 1, im, 1, 1, 1, kord)
+        ref = oracle.map1_ppm(pe1, q1, pe2, iv=1, kord=kord)
+        assert_close(q2.reshape(im, kn, order="F"), ref, RTOL, f"map1_ppm kord{kord}")
+    # cs_profile
+    a4 = np.zeros((4, im, km), order="F")
+    a4[0] = q1
+    fa.cs_profile(np.zeros(im), a4, np.asfortranarray(dp1), km, 1, im, 0, 9)
+    ref = oracle.cs_profile(q1, dp1, iv=0, kord=9)
+    assert_close(a4, ref, RTOL, "cs_profile", floor=1e-300)
+
+
+def test_sphere_tracer_2d_1L(gpu_ready):
+    case = cases.make_case("6xC16L4Q2", n=16, npz=4, nq=2, ntiles=6, courant=1.3,
+                           delp_kind=1, delp_amp=(0.05, 0.1), seed=8)
+    a, ns, ref = _run_tracer(case, "1L")
+    assert ns.max() >= 2
+    _check_all(a, ns, ref, "6-tile")
