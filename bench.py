@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Headline measurement: FV3 tracer transport + vertical remap on one MI355X.
+
+Workload: one C768 tile (768x768, 3-cell halo), 127 levels, 30 tracers + pt.
+One step = tracer_2d_1L (hord 8, per-level nsplit(k) from a jet wind
+profile) followed by the tracer (kord 9) and pt (kord 10) remap of
+Lagrangian_to_Eulerian, with every input already resident in HBM.  The
+timed form is the fused FP32-bulk form (FP64 thickness/pressure); the plain
+FP64 form is timed beside it, and the FP32 result is compared with the FP64
+one.  The CPU baseline is the FP64 CPU oracle on a bounded sample of the same
+workload (one tracer, every level, the full tile) on one host core.
+
+Prints ONE JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-tracer updates/s C768L127x30 tracers: FP32-batched vs plain FP64 @≤1e-5; %HBM"
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def device_inputs(case, torch, dtype):
+    """Flat device tensors in the Fortran layouts the C-ABI takes."""
+    f = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a.ravel(order="F"))).to("cuda", dt)
+    bases = torch.stack([torch.from_numpy(np.ascontiguousarray(b.T)) for b in case.meta["q_bases"][0]])
+    lev = torch.from_numpy(case.meta["lev"])
+    q = (bases.to("cuda", torch.float64)[:, None, :, :] * lev.to("cuda")[None, :, None, None]).to(dtype)
+    d = dict(q_in=q.reshape(-1).contiguous(), pt=f(case.pt, dtype), dp1=f(case.dp1, torch.float64),
+             cx=f(case.cx, dtype), cy=f(case.cy, dtype), mfx=f(case.mfx, dtype),
+             mfy=f(case.mfy, dtype), ak=f(case.ak, torch.float64), bk=f(case.bk, torch.float64))
+    d["q_out"] = torch.zeros_like(d["q_in"])
+    d["delp_out"] = torch.zeros_like(d["dp1"])
+    return d
+
+
+def nsplit_of(case):
+    """nsplit(k) = int(1 + cmax(k)) of tracer_2d_1L, from the case arrays."""
+    ng, n = 3, case.n
+    s5 = case.grids[0].sin_sg[ng:ng + n, ng:ng + n, 4]
+    out = []
+    for k in range(case.npz):
+        ax = np.abs(case.cx[:n, ng:ng + n, k])
+        ay = np.abs(case.cy[ng:ng + n, :n, k])
+        m = np.maximum(ax, ay)
+        c = m.max() if (k + 1) < case.npz // 6 else (m + 1.0 - s5).max()
+        out.append(int(1.0 + c))
+    return np.array(out)
+
+
+def run_form(fa, ctx, form, d, case, steps, warmup, cap, torch, timing=False):
+    stream = torch.cuda.current_stream().cuda_stream
+    t_adv, t_rem = [], []
+
+    def one():
+        fa.step_dev(ctx, form, case.nq, d["q_in"], d["q_out"], d["pt"], d["dp1"], d["delp_out"],
+                    d["cx"], d["cy"], d["mfx"], d["mfy"], d["ak"], d["bk"], case.ptop, case.hord,
+                    case.kord_tr, case.kord_tm, 184.0, cap, stream)
+        d["q_in"], d["q_out"] = d["q_out"], d["q_in"]
+
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+        if timing:
+            a, r, p = fa.step_timing(ctx)
+            t_adv.append(a)
+            t_rem.append(r)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return ms, t_adv, t_rem
+
+
+def cpu_baseline(case):
+    """FP64 oracle on the full tile, every level, one tracer (+ pt remap)."""
+    from oracle import oracle
+    import copy
+    c1 = copy.copy(case)
+    c1.nq = 1
+    c1.q = np.asfortranarray(case.meta["q_bases"][0][0][:, :, None, None] *
+                             case.meta["lev"][None, None, :, None])
+    t0 = time.perf_counter()
+    adv = oracle.tracer_2d(c1, "1L")
+    oracle.remap_tracers(c1, adv["q"], c1.dp1, c1.pt)
+    t = time.perf_counter() - t0
+    cells = case.n * case.n * case.npz * 1
+    return dict(value=cells / t / 1e9, unit="Gcell-tracer updates/s", cores=1, kind="port",
+                sample=f"FP64 oracle (C restatement), full C{case.n} tile, {case.npz} levels, "
+                       f"1 tracer (+pt remap), one step in {t:.1f} s on one host core")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=768)
+    ap.add_argument("--npz", type=int, default=127)
+    ap.add_argument("--nq", type=int, default=30)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "the multi-GPU layout (one cube tile per GPU) is described in DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fv3atm_amd as fa
+    from fv3atm_amd import cases
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    t0 = time.perf_counter()
+    case = cases.config(2, n=args.n, npz=args.npz, nq=args.nq, build_q=False)
+    nspl = nsplit_of(case)
+    S = int(nspl.sum())
+    N, L, Q = case.n, case.npz, case.nq
+    log(f"case C{N} L{L} Q{Q} built in {time.perf_counter() - t0:.1f}s; nsplit histogram "
+        f"{dict(zip(*np.unique(nspl, return_counts=True)))} S={S}")
+    cap = int(nspl.max())
+    out = {}
+
+    # ---- plain FP64 form (timing + reference for the FP32 error)
+    ref64 = None
+    if not args.no_plain:
+        ctx64 = fa.Context(case.grids, L)
+        d64 = device_inputs(case, torch, torch.float64)
+        q0_64 = d64["q_in"].clone()
+        pt0 = d64["pt"].clone()
+        fa.step_dev(ctx64, fa.FORM_PLAIN64, Q, d64["q_in"], d64["q_out"], d64["pt"], d64["dp1"],
+                    d64["delp_out"], d64["cx"], d64["cy"], d64["mfx"], d64["mfy"], d64["ak"],
+                    d64["bk"], case.ptop, case.hord, case.kord_tr, case.kord_tm, 184.0, cap)
+        torch.cuda.synchronize()
+        ref64 = dict(q=d64["q_out"].clone(), pt=d64["pt"].clone(), delp=d64["delp_out"].clone(),
+                     q0=q0_64)
+        d64["pt"].copy_(pt0)
+        del pt0
+        ms64, _, _ = run_form(fa, ctx64, fa.FORM_PLAIN64, d64, case, args.plain_steps, 0, cap, torch)
+        log(f"plain FP64: {ms64:.2f} ms/step")
+        out["plain64_ms_per_step"] = round(ms64, 3)
+        del d64
+        ctx64.close()
+        torch.cuda.empty_cache()
+
+    # ---- fused FP32 form
+    ctx = fa.Context(case.grids, L)
+    d = device_inputs(case, torch, torch.float32)
+    if ref64 is not None:  # one step from the same state, compared with FP64
+        fa.step_dev(ctx, fa.FORM_FUSED32, Q, d["q_in"], d["q_out"], d["pt"], d["dp1"], d["delp_out"],
+                    d["cx"], d["cy"], d["mfx"], d["mfy"], d["ak"], d["bk"], case.ptop, case.hord,
+                    case.kord_tr, case.kord_tm, 184.0, cap)
+        torch.cuda.synchronize()
+        nid = N + 6
+        shp = (Q, L, nid, nid)
+        g32 = d["q_out"].view(shp)[:, :, 3:-3, 3:-3].double()
+        g64 = ref64["q"].view(shp)[:, :, 3:-3, 3:-3]
+        num = torch.linalg.vector_norm((g32 - g64).reshape(Q, -1), dim=1)
+        den = torch.linalg.vector_norm(g64.reshape(Q, -1), dim=1)
+        rel = (num / den).cpu().numpy()
+        linf = ((g32 - g64).abs().amax(dim=(1, 2, 3)) / g64.abs().amax(dim=(1, 2, 3))).cpu().numpy()
+        area = torch.from_numpy(np.ascontiguousarray(case.grids[0].area.T)).cuda()[3:-3, 3:-3]
+        delp = ref64["delp"].view(L, nid, nid)[:, 3:-3, 3:-3]
+        dp1 = d["dp1"].view(L, nid, nid)[:, 3:-3, 3:-3]
+        q0 = ref64["q0"].view(shp)[:, :, 3:-3, 3:-3]
+        m0 = (q0 * dp1 * area).sum(dim=(1, 2, 3))
+        dm64 = (g64 * delp * area).sum(dim=(1, 2, 3)) - m0
+        dm32 = (g32 * delp * area).sum(dim=(1, 2, 3)) - m0
+        mass_err = ((dm32 - dm64).abs() / m0.abs()).max().item()
+        out["fp32_rel_l2_max"] = float(rel.max())
+        out["fp32_linf_rel_max"] = float(linf.max())
+        out["fp32_mass_err_per_step"] = float(mass_err)
+        log(f"FP32 vs FP64: rel L2 max {rel.max():.2e}, Linf rel max {linf.max():.2e}, "
+            f"mass error {mass_err:.2e}")
+        del ref64, g32, g64
+        torch.cuda.empty_cache()
+        d = device_inputs(case, torch, torch.float32)
+    fa.set_timing(ctx, True)
+    ms, t_adv, t_rem = run_form(fa, ctx, fa.FORM_FUSED32, d, case, args.steps, args.warmup, cap,
+                                torch, timing=True)
+    log(f"fused FP32: {ms:.2f} ms/step (transport {np.mean(t_adv):.2f} ms, remap {np.mean(t_rem):.2f} ms)")
+
+    cells = N * N * L * Q
+    value = cells / (ms * 1e-3) / 1e9
+    w = 4
+    b_adv = S * N * N * (2 * w * Q + 24)
+    b_rem = N * N * ((Q + 1) * 2 * w * L + 16 * (L + 1))
+    ta = float(np.mean(t_adv)) * 1e-3
+    roof = dict(bound="hbm", achieved=round(b_adv / ta / 1e9, 1), peak=HBM_PEAK / 1e9,
+                unit="GB/s", frac=round(b_adv / ta / HBM_PEAK, 4), traffic=None)
+    out_line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-tracer updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"C{N} tile L{L} x {Q} tracers + pt: tracer_2d_1L (hord 8, "
+                               f"per-level nsplit, S={S}) + Lagrangian_to_Eulerian tracer/pt remap "
+                               f"(kord 9/10), fused FP32 form",
+                   "n": N, "npz": L, "nq": Q, "nsplit_sum": S, "nsplit_max": cap,
+                   "form": "fused32"},
+        "roofline": roof,
+        "transport_ms": round(ta * 1e3, 3), "remap_ms": round(float(np.mean(t_rem)), 3),
+        "hbm_frac_step": round((b_adv + b_rem) / (ms * 1e-3) / HBM_PEAK, 4),
+        "algorithmic_bytes_per_step": b_adv + b_rem,
+    }
+    out_line.update(out)
+    if "plain64_ms_per_step" in out:
+        out_line["speedup_vs_plain64"] = round(out["plain64_ms_per_step"] / ms, 3)
+    ctx.close()
+    del d
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        cb = cpu_baseline(case)
+        log(f"cpu baseline {cb['value']:.4f} Gcell-tracer updates/s")
+        out_line["cpu_baseline"] = cb
+    else:
+        out_line["cpu_baseline"] = None
+    print(json.dumps(out_line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

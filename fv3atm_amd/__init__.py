@@ -156,3 +156,17 @@ def step_dev(ctx: Context, form, nq, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx
                                   ptr(delp_out), ptr(cx), ptr(cy), ptr(mfx), ptr(mfy), ptr(ak),
                                   ptr(bk), ptop, hord, kord_tr, kord_tm, t_min, nsplit_cap,
                                   stream), "fv3_step_dev")
+
+
+def set_timing(ctx: Context, on: bool):
+    """Record device times of the transport passes and the remap of each
+    fused step (hipEvents on the step's stream)."""
+    check(_ffi.lib().fv3_ctx_set_timing(ctx.handle, int(on)), "fv3_ctx_set_timing")
+
+
+def step_timing(ctx: Context):
+    """(transport_ms, remap_ms, passes) of the last timed fused step."""
+    a, r, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    check(_ffi.lib().fv3_step_timing(ctx.handle, ctypes.byref(a), ctypes.byref(r), ctypes.byref(p)),
+          "fv3_step_timing")
+    return a.value, r.value, p.value

@@ -46,6 +46,8 @@ SIGNATURES = {
     "fv3_cs_profile": (I, [P, P, P, I, I, I, I, I]),
     "fv3_lagrangian_to_eulerian_tracers": (I, [P, I, I, P, P, D, P, P, P, P, I, D, I]),
     "fv3_step_dev": (I, [P, I, I, P, P, P, P, P, P, P, P, P, P, P, D, I, I, I, D, I, P]),
+    "fv3_ctx_set_timing": (I, [P, I]),
+    "fv3_step_timing": (I, [P, DP, DP, IP]),
 }
 
 _lib = None

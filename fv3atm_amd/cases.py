@@ -217,7 +217,8 @@ def _tracer_params(nq: int, seed: int, tile: int):
 def make_case(name: str, n: int, npz: int, nq: int, *, ntiles: int = 1, tile: int = 1,
               wind: str = "solid", courant: float = 0.4, jet: bool = False,
               delp_kind: int = 0, delp_amp=(0.1,), seed: int = 0, q_split: int = 0,
-              with_pt: bool = False, tilt: float = math.pi / 4, tracers=None) -> Case:
+              with_pt: bool = False, tilt: float = math.pi / 4, tracers=None,
+              build_q: bool = True) -> Case:
     """Build a case.  ``courant`` is the target max |Courant| over the tile
     interior (for ``jet`` the max over levels)."""
     tiles = list(range(1, 7)) if ntiles == 6 else [tile]
@@ -249,22 +250,27 @@ def make_case(name: str, n: int, npz: int, nq: int, *, ntiles: int = 1, tile: in
     if tracers is None:
         tracers = _tracer_params(nq, seed, tiles[0] if ntiles == 1 else 1)
     nd = n + 2 * NG
-    q = np.zeros((nd, nd, npz, nq, len(grids)), dtype=np.float64, order="F")
     lev = 1.0 + 0.5 * np.arange(npz) / max(npz, 1)
-    for t, g in enumerate(grids):
-        for iq, (knd, pr) in enumerate(tracers):
-            base = tracer2d(g, knd, pr)
-            for k in range(npz):
-                q[:, :, k, iq, t] = base * lev[k]
-    if ntiles == 1:
-        q = np.asfortranarray(q[..., 0])
+    bases = [[tracer2d(g, knd, pr) for (knd, pr) in tracers] for g in grids]
+    if build_q:
+        q = np.zeros((nd, nd, npz, nq, len(grids)), dtype=np.float64, order="F")
+        for t in range(len(grids)):
+            for iq in range(nq):
+                q[:, :, :, iq, t] = bases[t][iq][:, :, None] * lev[None, None, :]
+        if ntiles == 1:
+            q = np.asfortranarray(q[..., 0])
+    else:
+        q = None  # q(i,j,k,iq) = bases[tile][iq](i,j) * lev(k); assemble where it is used
     pt = None
     if with_pt:
         pmid3 = PTOP + np.cumsum(dp1, axis=2) - 0.5 * dp1
-        pt = np.asfortranarray(np.maximum(184.0, 300.0 * (pmid3 / PS) ** (287.04 * 6.5e-3 / 9.80665)))
+        # smooth profile from 184 K at the model top to 300 K at the surface
+        # (a floor clamp would put a kink in every column)
+        pt = np.asfortranarray(184.0 + 116.0 * (pmid3 / PS) ** (287.04 * 6.5e-3 / 9.80665))
     return Case(name=name, n=n, npz=npz, nq=nq, ntiles=ntiles, grids=grids, q=q, dp1=dp1,
                 cx=cx, cy=cy, mfx=mfx, mfy=mfy, ak=ak, bk=bk, ptop=PTOP, pt=pt,
-                q_split=q_split, dt=dt, meta=dict(speed=speed, wind=wind, courant=courant))
+                q_split=q_split, dt=dt, meta=dict(speed=speed, wind=wind, courant=courant,
+                                                  q_bases=bases, lev=lev))
 
 
 def config(idx: int, **over) -> Case:

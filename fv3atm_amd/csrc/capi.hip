@@ -235,10 +235,13 @@ int fv3_ctx_destroy(fv3_ctx *h) {
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   (void)hipFree(c.g64);
   (void)hipFree(c.g32);
+  (void)hipFree(c.gfac32);
   (void)hipFree(c.halo_dst);
   (void)hipFree(c.halo_src);
   (void)hipFree(c.scratch);
   (void)hipFree(c.stage);
+  for (auto &e : c.ev)
+    if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
   return 0;
@@ -274,6 +277,27 @@ int fv3_ctx_set_grid(fv3_ctx *h, int tile, const fv3_grid *g) {
     tmp.resize(x.n);
     for (long long k = 0; k < x.n; k++) tmp[k] = (float)x.src[k];
     FV3_HIP(hipMemcpy(x.d32, tmp.data(), sizeof(float) * x.n, hipMemcpyHostToDevice));
+  }
+  if (tile == 1) {  // FP32 combined factors for the fused FP32 form (single tile)
+    const int nid = c.d.nid, njd = c.d.njd;
+    const long long nx = (long long)(nid + 1) * njd, ny = (long long)nid * (njd + 1);
+    std::vector<float> f(2 * nx + 2 * ny, 0.0f);
+    for (int j = 0; j < njd; j++)
+      for (int i = 1; i < nid; i++) {  // x-face i between cells i-1 and i (0-based from isd)
+        const long long fo = (long long)j * (nid + 1) + i;
+        const double dyv = g->dy[fo];
+        f[fo] = (float)(g->dxa[(long long)j * nid + i - 1] * dyv * g->sin_sg[2 * s2 + (long long)j * nid + i - 1]);
+        f[nx + fo] = (float)(g->dxa[(long long)j * nid + i] * dyv * g->sin_sg[0 * s2 + (long long)j * nid + i]);
+      }
+    for (int j = 1; j < njd; j++)
+      for (int i = 0; i < nid; i++) {  // y-face j between cells j-1 and j
+        const long long fo = (long long)j * nid + i;
+        const double dxv = g->dx[fo];
+        f[2 * nx + fo] = (float)(g->dya[(long long)(j - 1) * nid + i] * dxv * g->sin_sg[3 * s2 + (long long)(j - 1) * nid + i]);
+        f[2 * nx + ny + fo] = (float)(g->dya[(long long)j * nid + i] * dxv * g->sin_sg[1 * s2 + (long long)j * nid + i]);
+      }
+    if (!c.gfac32) FV3_HIP(hipMalloc(&c.gfac32, sizeof(float) * f.size()));
+    FV3_HIP(hipMemcpy(c.gfac32, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
   }
   if (tile == 1 || g->da_min < c.da_min || c.da_min == 0.0) c.da_min = g->da_min;
   unsigned corners = (g->sw_corner ? 1u : 0u) | (g->se_corner ? 2u : 0u) | (g->ne_corner ? 4u : 0u) |
@@ -631,6 +655,32 @@ int fv3_step_dev(fv3_ctx *h, int form, int nq, const void *q_in, void *q_out, vo
   }
   c.stream = saved;
   return rc;
+}
+
+int fv3_ctx_set_timing(fv3_ctx *h, int on) {
+  if (!h) return set_error("NULL context");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (on && !c.ev[0])
+    for (auto &e : c.ev) FV3_HIP(hipEventCreate(&e));
+  c.timing = on != 0;
+  c.ev_valid = false;
+  return 0;
+}
+
+int fv3_step_timing(fv3_ctx *h, double *transport_ms, double *remap_ms, int *passes) {
+  if (!h || !transport_ms || !remap_ms || !passes) return set_error("fv3_step_timing: NULL argument");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.ev_valid) return set_error("no timed fused step recorded (fv3_ctx_set_timing)");
+  float a = 0.f, b = 0.f;
+  FV3_HIP(hipEventSynchronize(c.ev[3]));
+  FV3_HIP(hipEventElapsedTime(&a, c.ev[0], c.ev[1]));
+  FV3_HIP(hipEventElapsedTime(&b, c.ev[2], c.ev[3]));
+  *transport_ms = a;
+  *remap_ms = b;
+  *passes = c.last_passes;
+  return 0;
 }
 
 }  // extern "C"

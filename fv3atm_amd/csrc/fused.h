@@ -1,9 +1,21 @@
-// fused.h -- the fused (one kernel per sub-cycle) transport and the step
-// driver of the device-resident forms.
+// fused.h -- the fused transport kernel launchers and the step driver of the
+// device-resident forms.
 #pragma once
+#include "fused_tp.h"
 #include "fv3_internal.h"
 
 namespace fv3 {
+
+constexpr int FUSED64_WAVES = 4;  // tracers in flight per workgroup, FP64
+constexpr int FUSED32_WAVES = 8;  // tracers in flight per workgroup, FP32
+
+int launch_fused_f64(const FusedArgs<double> &a, int nblk, hipStream_t s);
+int launch_fused_f32(const FusedArgs<float> &a, int nblk, hipStream_t s);
+
+// per-level sub-cycle counts on the device (cmax, nsplit; no scaling)
+template <typename T>
+int device_nsplit(Ctx &c, const T *cx, const T *cy, bool per_level, int q_split, T *cmax,
+                  int *nsplit, T *frac);
 
 int fused_step(Ctx &c, bool fp32, int nq, const void *q_in, void *q_out, void *pt,
                const double *dp1, double *delp_out, const void *cx, const void *cy,

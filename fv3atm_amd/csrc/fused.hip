@@ -1,9 +1,141 @@
+// fused.hip -- FP64 instantiation of the fused transport kernel and the
+// device-resident advection + remap step driver.
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
 #include "fused.h"
+#include "fused_kernel.h"
+
 namespace fv3 {
+
+int launch_fused_f64(const FusedArgs<double> &a, int nblk, hipStream_t s) {
+  dim3 g((unsigned)nblk, (unsigned)a.npz, 1), b(FT_LANES * FUSED64_WAVES);
+  hipLaunchKernelGGL((k_fused_tp<double, FUSED64_WAVES>), g, b, 0, s, a);
+  FV3_HIP(hipGetLastError());
+  return 0;
+}
+
+__global__ void k_nsplit_check(const int *nsplit, int npz, int cap, int *flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int bad = 0;
+    for (int k = 0; k < npz; k++) bad |= nsplit[k] > cap || nsplit[k] < 1;
+    *flag = bad;
+  }
+}
+
+template <typename T>
+static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const double *dp1,
+                        double *delp_out, const T *cx, const T *cy, const T *mfx, const T *mfy,
+                        const double *ak, const double *bk, double ptop, int hord, int kord_tr,
+                        int kord_tm, double t_min, int nsplit_cap) {
+  const Dims &d = c.d;
+  const int npz = c.npz;
+  hipStream_t s = c.stream;
+  if (c.ntiles != 1) return set_error("the fused forms take a single-tile context");
+  if (hord < 7 || hord > 13) return set_error("hord=%d: only the monotone orders 7..13 are implemented", hord);
+  // scratch: cmax, frac, nsplit, flag, then q_tmp
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    size_t o = off;
+    off += (b + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_cmax = take(sizeof(T) * npz), o_frac = take(sizeof(T) * npz);
+  const size_t o_ns = take(sizeof(int) * npz), o_flag = take(sizeof(int));
+  const size_t head = off;
+  char *base = (char *)ctx_stage(c, head);
+  if (!base) return 1;
+  T *cmax = (T *)(base + o_cmax), *frac = (T *)(base + o_frac);
+  int *nspl = (int *)(base + o_ns), *flag = (int *)(base + o_flag);
+  if (device_nsplit<T>(c, cx, cy, true, 0, cmax, nspl, frac)) return 1;
+  int cap = nsplit_cap;
+  if (cap <= 0) {
+    std::vector<int> h(npz);
+    FV3_HIP(hipMemcpyAsync(h.data(), nspl, sizeof(int) * npz, hipMemcpyDeviceToHost, s));
+    FV3_HIP(hipStreamSynchronize(s));
+    cap = *std::max_element(h.begin(), h.end());
+    if (cap < 1) return set_error("nsplit < 1 from a non-finite Courant number");
+  }
+  const long long qsize = d.s2 * npz * nq;
+  T *q_tmp = nullptr;
+  if (cap > 1) {
+    q_tmp = (T *)ctx_scratch(c, sizeof(T) * qsize);
+    if (!q_tmp) return 1;
+  }
+  FusedArgs<T> a{};
+  a.is = d.is; a.ie = d.ie; a.js = d.js; a.je = d.je;
+  a.isd = d.isd; a.ied = d.ied; a.jsd = d.jsd; a.jed = d.jed;
+  a.npx = d.npx; a.npy = d.npy; a.nid = d.nid;
+  a.s2 = d.s2; a.sx = d.sx; a.sy = d.sy; a.smx = d.smx; a.smy = d.smy;
+  a.corners = d.bounded ? 0u : d.corners;
+  a.edge = d.edge;
+  a.npz = npz; a.nq = nq; a.hord = hord; a.cap = cap;
+  const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
+  a.H = std::min(nj, 90);
+  a.nstrip = (ni + FT_BX - 1) / FT_BX;
+  a.nvblk = (nj + a.H - 1) / a.H;
+  a.nsplit = nspl;
+  a.q_in = q_in; a.q_out = q_out; a.q_tmp = q_tmp;
+  a.cx = cx; a.cy = cy; a.mfx = mfx; a.mfy = mfy; a.dp1 = dp1;
+  const GridDev<T> &g = grid_of<T>(c);
+  a.area = g.area; a.rarea = g.rarea; a.dxa = g.dxa; a.dya = g.dya; a.dx = g.dx; a.dy = g.dy;
+  a.sin_sg = g.sin_sg; a.ms2 = g.s2;
+  if (sizeof(T) == 4) {
+    a.gxp = (const T *)c.gfac32;
+    a.gxm = a.gxp + (long long)(d.nid + 1) * d.njd;
+    a.gyp = a.gxm + (long long)(d.nid + 1) * d.njd;
+    a.gym = a.gyp + (long long)d.nid * (d.njd + 1);
+  }
+  const int nblk = a.nstrip * a.nvblk;
+  if (c.timing) FV3_HIP(hipEventRecord(c.ev[0], s));
+  for (int it = 1; it <= cap; it++) {
+    a.it = it;
+    int rc;
+    if constexpr (std::is_same<T, double>::value)
+      rc = launch_fused_f64(a, nblk, s);
+    else
+      rc = launch_fused_f32(a, nblk, s);
+    if (rc) return rc;
+  }
+  if (nsplit_cap > 0) {
+    hipLaunchKernelGGL(k_nsplit_check, dim3(1), dim3(1), 0, s, nspl, npz, cap, flag);
+    FV3_HIP(hipGetLastError());
+  }
+  if (c.timing) FV3_HIP(hipEventRecord(c.ev[1], s));
+  // remap: source thickness is dp1, target lands in delp_out
+  FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
+  std::vector<int> kt(nq, kord_tr);
+  if (c.timing) FV3_HIP(hipEventRecord(c.ev[2], s));
+  if (remap_tracers_plain<T>(c, npz, nq, ak, bk, ptop, delp_out, q_out, pt, kt.data(), kord_tm, t_min, 0))
+    return 1;
+  if (c.timing) {
+    FV3_HIP(hipEventRecord(c.ev[3], s));
+    c.last_passes = cap;
+    c.ev_valid = true;
+  }
+  if (nsplit_cap > 0) {
+    int hf = 0;
+    FV3_HIP(hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    FV3_HIP(hipStreamSynchronize(s));
+    if (hf) return set_error("a level needs more sub-cycles than nsplit_cap=%d", nsplit_cap);
+  }
+  return 0;
+}
+
 int fused_step(Ctx &c, bool fp32, int nq, const void *q_in, void *q_out, void *pt,
                const double *dp1, double *delp_out, const void *cx, const void *cy,
                const void *mfx, const void *mfy, const double *ak, const double *bk, double ptop,
                int hord, int kord_tr, int kord_tm, double t_min, int nsplit_cap) {
-  return set_error("fused forms not built");
+  if (fp32)
+    return fused_step_t<float>(c, nq, (const float *)q_in, (float *)q_out, (float *)pt, dp1, delp_out,
+                               (const float *)cx, (const float *)cy, (const float *)mfx,
+                               (const float *)mfy, ak, bk, ptop, hord, kord_tr, kord_tm, t_min,
+                               nsplit_cap);
+  return fused_step_t<double>(c, nq, (const double *)q_in, (double *)q_out, (double *)pt, dp1, delp_out,
+                              (const double *)cx, (const double *)cy, (const double *)mfx,
+                              (const double *)mfy, ak, bk, ptop, hord, kord_tr, kord_tm, t_min,
+                              nsplit_cap);
 }
+
 }  // namespace fv3

@@ -56,6 +56,9 @@ struct Ctx {
   float *g32 = nullptr;
   GridDev<double> grid64{};
   GridDev<float> grid32{};
+  // FP32 upwind area-flux factors of tile 1: gxp, gxm (isd:ied+1, jsd:jed),
+  // gyp, gym (isd:ied, jsd:jed+1)
+  float *gfac32 = nullptr;
   bool grid_set = false;
   // six-tile edge-halo table (dst, src) flat offsets within a tile-stacked slab set
   long long *halo_dst = nullptr, *halo_src = nullptr;
@@ -66,6 +69,11 @@ struct Ctx {
   void *stage = nullptr;  // staging area for host-pointer entry points
   size_t stage_bytes = 0;
   std::mutex mu;
+  // optional phase timing of fv3_step_dev (hipEvents on the step's stream)
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int last_passes = 0;
+  bool ev_valid = false;
 };
 
 // error reporting (thread-local last error string)

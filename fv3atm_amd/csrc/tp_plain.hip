@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <vector>
 
+#include "fused.h"
 #include "fv3_internal.h"
 
 namespace fv3 {
@@ -115,6 +116,25 @@ __global__ void k_nsplit(const T *cmax, int npz, int per_level, int q_split, int
     frac[k] = T(1) / T(n);
   }
 }
+
+template <typename T>
+int device_nsplit(Ctx &c, const T *cx, const T *cy, bool per_level, int q_split, T *cmax,
+                  int *nsplit, T *frac) {
+  const Dims d = c.d;
+  const int npz = c.npz, nt = c.ntiles;
+  FV3_HIP(hipMemsetAsync(cmax, 0, sizeof(T) * npz, c.stream));
+  dim3 b(256), gr((unsigned)std::min(64, (d.ie - d.is + 1) * (d.je - d.js + 1) / 256 + 1), 1,
+                  (unsigned)(npz * nt));
+  hipLaunchKernelGGL(k_cmax<T>, gr, b, 0, c.stream, cx, cy, grid_of<T>(c).sin_sg, d, npz,
+                     grid_of<T>(c).s2, cmax);
+  FV3_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_nsplit<T>, dim3(1), dim3(1), 0, c.stream, cmax, npz, per_level ? 1 : 0,
+                     q_split, nsplit, frac);
+  FV3_HIP(hipGetLastError());
+  return 0;
+}
+template int device_nsplit<double>(Ctx &, const double *, const double *, bool, int, double *, int *, double *);
+template int device_nsplit<float>(Ctx &, const float *, const float *, bool, int, float *, int *, float *);
 
 template <typename T>
 static size_t align_up(size_t b) {

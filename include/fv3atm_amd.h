@@ -129,6 +129,14 @@ int fv3_step_dev(fv3_ctx *ctx, int form, int nq, const void *q_in, void *q_out, 
                  double ptop, int hord, int kord_tr, int kord_tm, double t_min,
                  int nsplit_cap, void *stream);
 
+/* Instrumentation (no counterpart upstream, where the "FV3 Dycore" clock of
+ * atmos_model.F90:845-847 times the whole dycore): when enabled, each fused
+ * fv3_step_dev records hipEvents on its stream around the transport passes
+ * and around the remap; fv3_step_timing returns the last step's device times
+ * and the number of transport passes launched. */
+int fv3_ctx_set_timing(fv3_ctx *ctx, int on);
+int fv3_step_timing(fv3_ctx *ctx, double *transport_ms, double *remap_ms, int *passes);
+
 #ifdef __cplusplus
 }
 #endif

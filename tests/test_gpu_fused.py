@@ -1,0 +1,109 @@
+"""GPU parity of the device-resident advection + remap step (fv3_step_dev).
+
+FUSED64 and PLAIN64 must match the FP64 oracle (tracer_2d_1L followed by the
+tracer/pt remap) to 1e-12 relative per interior cell; FUSED32 must stay
+within 1e-5 relative L2 per tracer and conserve tracer mass to 1e-6
+relative over the advection.
+"""
+import numpy as np
+import pytest
+import torch
+
+import fv3atm_amd as fa
+from fv3atm_amd import cases
+from oracle import oracle
+from tests._util import assert_close, interior, rel_l2
+
+pytestmark = pytest.mark.gpu
+NG = 3
+
+
+def _dev(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).to(
+        device="cuda", dtype=dtype)
+
+
+def run_step(case, form, nsplit_cap=0):
+    dt = torch.float32 if form == fa.FORM_FUSED32 else torch.float64
+    ctx = fa.Context(case.grids, case.npz)
+    q_in = _dev(case.q, dt)
+    q_out = torch.zeros_like(q_in)
+    pt = _dev(case.pt, dt) if case.pt is not None else None
+    dp1 = _dev(case.dp1, torch.float64)
+    delp_out = torch.zeros_like(dp1)
+    cx, cy, mfx, mfy = (_dev(getattr(case, k), dt) for k in ("cx", "cy", "mfx", "mfy"))
+    ak, bk = _dev(case.ak, torch.float64), _dev(case.bk, torch.float64)
+    fa.step_dev(ctx, form, case.nq, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx, mfy, ak, bk,
+                case.ptop, case.hord, case.kord_tr, case.kord_tm, 184.0, nsplit_cap)
+    torch.cuda.synchronize()
+    shape_q = case.q.shape
+    out = dict(q=q_out.double().cpu().numpy().reshape(shape_q, order="F"),
+               delp=delp_out.cpu().numpy().reshape(case.dp1.shape, order="F"))
+    if pt is not None:
+        out["pt"] = pt.double().cpu().numpy().reshape(case.pt.shape, order="F")
+    ctx.close()
+    return out
+
+
+def reference(case):
+    adv = oracle.tracer_2d(case, "1L")
+    q, delp, pt = oracle.remap_tracers(case, adv["q"], case.dp1, case.pt)
+    return dict(q=q, delp=delp, pt=pt, q_adv=adv["q"], nsplit=adv["nsplit"])
+
+
+CASES = {
+    "C48L32Q1": lambda: cases.config(0, with_pt=True),
+    "C24L12Q3jet": lambda: cases.make_case("C24L12Q3jet", n=24, npz=12, nq=3, jet=True,
+                                           courant=2.5, delp_kind=1, delp_amp=(0.05, 0.2),
+                                           seed=3, with_pt=True),
+    # two strips, two row blocks, 7 tracers (mapn_tracer branch), uneven nsplit
+    "C100L6Q7jet": lambda: cases.make_case("C100L6Q7", n=100, npz=6, nq=7, jet=True,
+                                           courant=1.9, delp_kind=1, delp_amp=(0.05, 0.15),
+                                           seed=11, with_pt=True),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("form", [fa.FORM_FUSED64, fa.FORM_PLAIN64])
+def test_step_fp64_parity(gpu_ready, name, form):
+    case = CASES[name]()
+    ref = reference(case)
+    print(name, "form", form, "nsplit", np.unique(ref["nsplit"]).tolist(), flush=True)
+    got = run_step(case, form)
+    assert_close(interior(got["q"]), interior(ref["q"]), 1e-12, f"{name} q", floor=1e-300)
+    assert_close(interior(got["delp"]), interior(ref["delp"]), 1e-12, f"{name} delp")
+    assert_close(interior(got["pt"]), interior(ref["pt"]), 1e-12, f"{name} pt")
+
+
+@pytest.mark.parametrize("hord", [10, 13])
+def test_step_fused64_hord(gpu_ready, hord):
+    case = CASES["C24L12Q3jet"]()
+    case.hord = hord
+    ref = reference(case)
+    got = run_step(case, fa.FORM_FUSED64)
+    assert_close(interior(got["q"]), interior(ref["q"]), 1e-12, f"hord{hord} q", floor=1e-300)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_step_fused32_error(gpu_ready, name):
+    case = CASES[name]()
+    ref = reference(case)
+    got = run_step(case, fa.FORM_FUSED32)
+    errs = []
+    for iq in range(case.nq):
+        e = rel_l2(interior(got["q"][..., iq]), interior(ref["q"][..., iq]))
+        errs.append(e)
+    print(name, "FP32 rel L2 per tracer", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) <= 1e-5
+    assert rel_l2(interior(got["pt"]), interior(ref["pt"])) <= 1e-6
+    assert_close(interior(got["delp"]), interior(ref["delp"]), 1e-12, "delp")
+
+
+def test_step_nsplit_cap(gpu_ready):
+    case = CASES["C24L12Q3jet"]()
+    ref = reference(case)
+    cap = int(ref["nsplit"].max())
+    got = run_step(case, fa.FORM_FUSED64, nsplit_cap=cap + 1)
+    assert_close(interior(got["q"]), interior(ref["q"]), 1e-12, "cap q", floor=1e-300)
+    with pytest.raises(RuntimeError, match="nsplit_cap"):
+        run_step(case, fa.FORM_FUSED64, nsplit_cap=cap - 1)
