@@ -236,3 +236,42 @@ def fillz(q, dp):
     im, km, nq = q.shape
     lib().fv3o_fillz(im, km, nq, _p(q), _p(dp))
     return q
+
+
+def xppm_line(q, c, iord=8, edge=True, dxa=None):
+    """One xppm row on a tile of n = len(q) - 6 interior cells (is=1, ie=n,
+    3-cell halo).  q, dxa: (n+6,), c: (n+1,).  Returns flux (n+1,)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    nd = q.shape[0]
+    n = nd - 2 * NG
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    dxa = np.ones(nd) if dxa is None else np.ascontiguousarray(dxa, dtype=np.float64)
+    flux = np.zeros(n + 1)
+    rc = lib().fv3o_xppm(_p(flux), _p(q), _p(c), iord, 1, n, 1 - NG, n + NG, 1, 1, 1, 1, n + 1,
+                         n + 1, _p(dxa), 0 if edge else 1, 0, 1.0)
+    if rc:
+        raise ValueError("xppm failed")
+    return flux
+
+
+def yppm_line(q, c, jord=8, edge=True, dya=None):
+    """yppm on a single column (the transpose of xppm_line)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    nd = q.shape[0]
+    n = nd - 2 * NG
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    dya = np.ones(nd) if dya is None else np.ascontiguousarray(dya, dtype=np.float64)
+    flux = np.zeros(n + 1)
+    rc = lib().fv3o_yppm(_p(flux), _p(q), _p(c), jord, 1, 1, 1, 1, 1, n, 1 - NG, n + NG, n + 1,
+                         n + 1, _p(dya), 0 if edge else 1, 0, 1.0)
+    if rc:
+        raise ValueError("yppm failed")
+    return flux
+
+
+def copy_corners(q, n, dir):
+    q = np.array(q, order="F", copy=True, dtype=np.float64)
+    g = GridC(None, None, None, None, None, None, None, None, None, 0.0, 1, 1, 1, 1, 0, 0)
+    bd = bounds(n)
+    lib().fv3o_copy_corners(_p(q), n + 1, n + 1, dir, ctypes.byref(bd), ctypes.byref(g))
+    return q

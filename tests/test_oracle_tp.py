@@ -1,0 +1,200 @@
+"""CPU pins of the transport half of the FP64 oracle (tp_core / fv_tracer2d
+restatement).  Each check is fixed by the mathematics, not by retyping a
+formula: exactness on polynomials, the discrete max principle of the
+monotone order, conservation, symmetry, and the nsplit rule.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fv3atm_amd import cases
+from oracle import oracle
+
+NG = 3
+
+
+def _line(n):
+    xi = np.arange(1 - NG, n + NG + 1, dtype=np.float64)  # centre of cell i is x = i
+    xf = np.arange(1, n + 2) - 0.5                          # face i is at x = i - 1/2
+    return xi, xf
+
+
+@pytest.mark.parametrize("iord", [8, 9, 10, 11, 13])
+@pytest.mark.parametrize("edge", [True, False])
+def test_ppm_exact_for_linear_data(iord, edge):
+    # the swept-region mean of a linear field is its value at the swept midpoint;
+    # on a uniform grid this also holds through the one-sided cube-edge formulas
+    n = 24
+    xi, xf = _line(n)
+    rng = np.random.default_rng(iord)
+    a, b = rng.normal(), rng.normal()
+    if iord in (9, 13):  # the positive-definite orders act on non-positive cells
+        a = 3.0 * n * (abs(b) + 1.0)
+    c = rng.uniform(-1, 1, n + 1)
+    f = oracle.xppm_line(a + b * xi, c, iord, edge=edge)
+    np.testing.assert_allclose(f, a + b * (xf - c / 2), rtol=0, atol=1e-13 * (abs(a) + abs(b) * n))
+
+
+def test_ppm_exact_for_quadratic_cell_means():
+    # away from limiter activity (monotone, convex data) the hord-8 face value
+    # is the 4th-order one, exact for a parabola's cell means
+    n = 24
+    xi, xf = _line(n)
+    xi, xf = xi + 10.0, xf + 10.0
+    qm = xi ** 2 + 1.0 / 12.0
+    rng = np.random.default_rng(1)
+    c = rng.uniform(-1, 1, n + 1)
+    c[np.abs(c) < 1e-3] = 0.5
+    f = oracle.xppm_line(qm, c, 8, edge=False)
+    lo, hi = np.minimum(xf, xf - c), np.maximum(xf, xf - c)
+    exact = (hi ** 3 - lo ** 3) / (3.0 * (hi - lo))
+    np.testing.assert_allclose(f, exact, rtol=1e-13)
+
+
+def test_ppm_unit_courant_takes_the_upwind_cell():
+    n = 20
+    rng = np.random.default_rng(2)
+    q = rng.uniform(0, 1, n + 2 * NG)
+    np.testing.assert_array_equal(oracle.xppm_line(q, np.ones(n + 1)), q[NG - 1:NG + n])
+    np.testing.assert_array_equal(oracle.xppm_line(q, -np.ones(n + 1)), q[NG:NG + n + 1])
+
+
+def test_yppm_is_xppm_transposed():
+    n = 20
+    rng = np.random.default_rng(3)
+    q = rng.uniform(0, 1, n + 2 * NG)
+    c = rng.uniform(-1, 1, n + 1)
+    d = rng.uniform(0.8, 1.2, n + 2 * NG)
+    for iord in (8, 10, 13):
+        np.testing.assert_array_equal(oracle.xppm_line(q, c, iord, dxa=d),
+                                      oracle.yppm_line(q, c, iord, dya=d))
+
+
+@pytest.mark.parametrize("iord", [8, 11])
+def test_monotone_order_max_principle(iord):
+    # 1-D flux-form update with uniform |c| <= 1 creates no new extrema
+    n = 30
+    rng = np.random.default_rng(4)
+    worst = 0.0
+    for _ in range(300):
+        q = (rng.uniform(0, 1, n + 2 * NG) > 0.5) + 0.02 * rng.uniform(0, 1, n + 2 * NG)
+        c = rng.uniform(-1, 1)
+        F = oracle.xppm_line(q, np.full(n + 1, c), iord, edge=False) * c
+        qn = q[NG:NG + n] - (F[1:] - F[:-1])
+        lo = np.minimum.reduce([q[NG - 1:NG - 1 + n], q[NG:NG + n], q[NG + 1:NG + 1 + n]])
+        hi = np.maximum.reduce([q[NG - 1:NG - 1 + n], q[NG:NG + n], q[NG + 1:NG + 1 + n]])
+        worst = max(worst, (lo - qn).max(), (qn - hi).max())
+    assert worst <= 1e-14
+
+
+def test_pert_ppm_iv1_is_the_ppm_monotonicity_constraint():
+    # after iv=1 the parabola (al, ar, a6 = -3(al+ar)) is monotone within the cell
+    rng = np.random.default_rng(5)
+    import ctypes
+    al = rng.normal(size=2000)
+    ar = rng.normal(size=2000)
+    a0 = np.zeros(2000)
+    oracle.lib().fv3o_pert_ppm(2000, oracle._p(a0), oracle._p(al), oracle._p(ar), 1)
+    s = np.linspace(0, 1, 41)[:, None]
+    prof = al + s * (ar - al + (-3 * (al + ar)) * (1 - s))  # q(s) - a0 with a6 = -3(al+ar)
+    d = np.diff(prof, axis=0)
+    mono = (d >= -1e-12).all(axis=0) | (d <= 1e-12).all(axis=0)
+    assert mono.all()
+
+
+def test_copy_corners_rotations():
+    # dir=1 then dir=2 fill the same corner blocks from the two adjacent strips;
+    # on a field that is symmetric under the diagonal swap both fills agree,
+    # and every corner value comes from an edge-halo cell (never another corner)
+    n = 10
+    nd = n + 2 * NG
+    rng = np.random.default_rng(6)
+    base = rng.uniform(size=(nd, nd))
+    sym = np.asfortranarray(base + base.T)
+    sym_c = sym.copy(order="F")
+    blocks = [(slice(0, NG), slice(0, NG)), (slice(n + NG, nd), slice(n + NG, nd))]
+    for s in blocks:
+        sym_c[s] = np.nan
+    q1 = oracle.copy_corners(sym_c, n, 1)
+    q2 = oracle.copy_corners(sym_c, n, 2)
+    for s in blocks:  # SW and NE blocks lie on the swap diagonal
+        np.testing.assert_array_equal(q1[s], q2[s].T)
+        assert np.isfinite(q1[s]).all()
+
+
+def test_constant_tracer_preserved_single_tile():
+    case = cases.make_case("C20L4Q1jet", n=20, npz=4, nq=1, jet=True, courant=2.2, seed=2,
+                           delp_kind=0, delp_amp=(0.0,))
+    case.q[...] = 0.37
+    r = oracle.tracer_2d(case, "1L")
+    assert r["nsplit"].max() >= 2
+    q = r["q"][NG:-NG, NG:-NG]
+    assert np.abs(q - 0.37).max() <= 1e-14
+
+
+def test_sphere_mass_conservation_and_constancy():
+    case = cases.make_case("6xC12L3Q2", n=12, npz=3, nq=2, ntiles=6, courant=1.6, seed=4,
+                           delp_kind=0, delp_amp=(0.0,))
+    case.q[:, :, :, 1, :] = 1.25
+    m0 = sum((case.q[NG:-NG, NG:-NG, :, 0, t] * case.dp1[NG:-NG, NG:-NG, :, t] *
+              case.grids[t].area[NG:-NG, NG:-NG, None]).sum() for t in range(6))
+    r = oracle.tracer_2d(case, "1L")
+    assert r["nsplit"].max() == 2
+    # dp1 returns as the thickness at the start of the last sub-cycle; the
+    # mass after the step is q * dp2, with dp2 = dp1 for these non-divergent winds
+    m1 = sum((r["q"][NG:-NG, NG:-NG, :, 0, t] * r["dp1"][NG:-NG, NG:-NG, :, t] *
+              case.grids[t].area[NG:-NG, NG:-NG, None]).sum() for t in range(6))
+    assert abs(m1 - m0) / m0 <= 1e-13
+    assert np.abs(r["q"][NG:-NG, NG:-NG, :, 1, :] - 1.25).max() <= 1e-13
+
+
+def test_nsplit_rule():
+    case = cases.make_case("C16L12jet", n=16, npz=12, nq=1, jet=True, courant=2.4, seed=1)
+    r = oracle.tracer_2d(case, "1L")
+    s5 = case.grids[0].sin_sg[NG:-NG, NG:-NG, 4]
+    expect = []
+    for k in range(case.npz):
+        m = np.maximum(np.abs(case.cx[:16, NG:-NG, k]), np.abs(case.cy[NG:-NG, :16, k]))
+        cm = m.max() if (k + 1) < case.npz // 6 else (m + 1.0 - s5).max()
+        expect.append(int(1.0 + cm))
+    np.testing.assert_array_equal(r["nsplit"], expect)
+    # the winds come back scaled by 1/nsplit(k)
+    for k in range(case.npz):
+        np.testing.assert_allclose(r["cx"][..., k], case.cx[..., k] / expect[k], rtol=1e-15)
+
+
+def test_tracer_2d_forced_split():
+    case = cases.make_case("C12L3", n=12, npz=3, nq=1, courant=0.5, q_split=3, seed=3)
+    r = oracle.tracer_2d(case, "2d")
+    assert (r["nsplit"] == 3).all()
+
+
+@pytest.mark.parametrize("nord", [0, 1, 2])
+def test_deln_flux_zero_on_constants_and_conservative(nord):
+    case = cases.make_case("C12L1", n=12, npz=1, nq=1, courant=0.6, seed=7, q_split=1)
+    case.q[...] = 2.0
+    r = oracle.tracer_2d(case, "2d", nord_tr=nord, trdm=0.15)
+    assert np.abs(r["q"][NG:-NG, NG:-NG] - 2.0).max() <= 1e-14
+    # damping on a noisy field reduces the interior variance
+    case2 = cases.make_case("C12L1", n=12, npz=1, nq=1, courant=1e-9, seed=7, q_split=1)
+    rng = np.random.default_rng(nord)
+    case2.q[...] = rng.uniform(size=case2.q.shape)
+    plain = oracle.tracer_2d(case2, "2d")
+    damped = oracle.tracer_2d(case2, "2d", nord_tr=nord, trdm=0.2)
+    v0 = plain["q"][NG:-NG, NG:-NG].var()
+    v1 = damped["q"][NG:-NG, NG:-NG].var()
+    assert v1 < v0
+
+
+def test_grid_area_and_halo_consistency():
+    n = 16
+    grids = [cases.make_grid(n, t) for t in range(1, 7)]
+    tot = sum(g.area[NG:-NG, NG:-NG].sum() for g in grids)
+    assert abs(tot / (4 * math.pi * cases.RADIUS ** 2) - 1) <= 1e-12
+    stack = np.concatenate([g.area.ravel(order="F") for g in grids])
+    for t in range(1, 7):
+        m = cases.halo_map(n, t).ravel(order="F")
+        idx = np.nonzero(m >= 0)[0]
+        a = grids[t - 1].area.ravel(order="F")[idx]
+        assert np.abs(a - stack[m[idx]]).max() / a.max() <= 1e-13
