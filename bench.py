@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", type=int, default=-1,
+                    help="fused launch configuration; -1 tunes over all of them first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -182,7 +184,8 @@ def main():
         m0 = (q0 * dp1 * area).sum(dim=(1, 2, 3))
         dm64 = (g64 * delp * area).sum(dim=(1, 2, 3)) - m0
         dm32 = (g32 * delp * area).sum(dim=(1, 2, 3)) - m0
-        mass_err = ((dm32 - dm64).abs() / m0.abs()).max().item()
+        mabs = (q0.abs() * dp1 * area).sum(dim=(1, 2, 3))
+        mass_err = ((dm32 - dm64).abs() / mabs).max().item()
         out["fp32_rel_l2_max"] = float(rel.max())
         out["fp32_linf_rel_max"] = float(linf.max())
         out["fp32_mass_err_per_step"] = float(mass_err)
@@ -192,6 +195,19 @@ def main():
         torch.cuda.empty_cache()
         d = device_inputs(case, torch, torch.float32)
     fa.set_timing(ctx, True)
+    launch = args.launch
+    if launch < 0:  # untimed tuning of the launch shape on this workload
+        best = None
+        for cfg in range(fa.N_LAUNCH_CFG):
+            fa.set_launch(ctx, cfg, 0)
+            _, ta_c, _ = run_form(fa, ctx, fa.FORM_FUSED32, d, case, 2, 1, cap, torch, timing=True)
+            t = float(np.min(ta_c))
+            log(f"launch cfg {cfg}: transport {t:.2f} ms")
+            if best is None or t < best[0]:
+                best = (t, cfg)
+        launch = best[1]
+    fa.set_launch(ctx, launch, 0)
+    out["launch_cfg"] = launch
     ms, t_adv, t_rem = run_form(fa, ctx, fa.FORM_FUSED32, d, case, args.steps, args.warmup, cap,
                                 torch, timing=True)
     log(f"fused FP32: {ms:.2f} ms/step (transport {np.mean(t_adv):.2f} ms, remap {np.mean(t_rem):.2f} ms)")

@@ -170,3 +170,12 @@ def step_timing(ctx: Context):
     check(_ffi.lib().fv3_step_timing(ctx.handle, ctypes.byref(a), ctypes.byref(r), ctypes.byref(p)),
           "fv3_step_timing")
     return a.value, r.value, p.value
+
+
+def set_launch(ctx: Context, cfg: int = 0, rows: int = 0, remap: int = 0):
+    """Kernel choices of the FP32 step: transport launch shape (cfg, rows;
+    results unchanged) and remap kernel (0 column-shared, 1 per column)."""
+    check(_ffi.lib().fv3_ctx_set_launch(ctx.handle, cfg, rows, remap), "fv3_ctx_set_launch")
+
+
+N_LAUNCH_CFG = 6

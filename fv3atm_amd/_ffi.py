@@ -48,6 +48,7 @@ SIGNATURES = {
     "fv3_step_dev": (I, [P, I, I, P, P, P, P, P, P, P, P, P, P, P, D, I, I, I, D, I, P]),
     "fv3_ctx_set_timing": (I, [P, I]),
     "fv3_step_timing": (I, [P, DP, DP, IP]),
+    "fv3_ctx_set_launch": (I, [P, I, I, I]),
 }
 
 _lib = None

@@ -29,6 +29,7 @@ HIP_SOURCES = {
     "capi.hip": ["-ffp-contract=off"],
     "fused.hip": ["-ffp-contract=off"],
     "fused32.hip": ["-ffp-contract=fast"],
+    "mapz_fast.hip": ["-ffp-contract=fast"],
 }
 
 

@@ -6,11 +6,11 @@
 
 namespace fv3 {
 
-constexpr int FUSED64_WAVES = 4;  // tracers in flight per workgroup, FP64
-constexpr int FUSED32_WAVES = 8;  // tracers in flight per workgroup, FP32
-
+// launch configurations (waves per workgroup x tracers per wave); cfg 0 is
+// the default, the others are the autotuner's candidates
+constexpr int FUSED32_NCFG = 6;
 int launch_fused_f64(const FusedArgs<double> &a, int nblk, hipStream_t s);
-int launch_fused_f32(const FusedArgs<float> &a, int nblk, hipStream_t s);
+int launch_fused_f32(const FusedArgs<float> &a, int nblk, int cfg, hipStream_t s);
 
 // per-level sub-cycle counts on the device (cmax, nsplit; no scaling)
 template <typename T>

@@ -6,12 +6,13 @@
 
 #include "fused.h"
 #include "fused_kernel.h"
+#include "mapz_fast.h"
 
 namespace fv3 {
 
 int launch_fused_f64(const FusedArgs<double> &a, int nblk, hipStream_t s) {
-  dim3 g((unsigned)nblk, (unsigned)a.npz, 1), b(FT_LANES * FUSED64_WAVES);
-  hipLaunchKernelGGL((k_fused_tp<double, FUSED64_WAVES>), g, b, 0, s, a);
+  dim3 g((unsigned)nblk, (unsigned)a.npz, 1), b(FT_LANES * 4);
+  hipLaunchKernelGGL((k_fused_tp<double, 4, 1>), g, b, 0, s, a);
   FV3_HIP(hipGetLastError());
   return 0;
 }
@@ -58,8 +59,22 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     if (cap < 1) return set_error("nsplit < 1 from a non-finite Courant number");
   }
   const long long qsize = d.s2 * npz * nq;
-  T *q_tmp = nullptr;
-  if (cap > 1) {
+  const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
+  // The FP32 form remaps out of place: the last transport pass of every
+  // level lands in q_tmp (the caller's q_out serves as the alternate pass
+  // buffer), and the remap writes q_out.
+  const bool fast_remap = std::is_same<T, float>::value && c.remap_mode == 0 && npz >= 8;
+  T *q_tmp = nullptr, *pt_src = nullptr;
+  RemapCols cols{};
+  if (fast_remap) {
+    const size_t b_q = ((sizeof(T) * qsize) + 255) & ~(size_t)255;
+    const size_t b_pt = pt ? ((sizeof(T) * d.s2 * npz) + 255) & ~(size_t)255 : 0;
+    char *sc = (char *)ctx_scratch(c, b_q + b_pt + remap_cols_bytes(npz, (long long)ni * nj));
+    if (!sc) return 1;
+    q_tmp = (T *)sc;
+    pt_src = pt ? (T *)(sc + b_q) : nullptr;
+    cols = remap_cols_carve(sc + b_q + b_pt, npz, (long long)ni * nj);
+  } else if (cap > 1) {
     q_tmp = (T *)ctx_scratch(c, sizeof(T) * qsize);
     if (!q_tmp) return 1;
   }
@@ -71,12 +86,13 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   a.corners = d.bounded ? 0u : d.corners;
   a.edge = d.edge;
   a.npz = npz; a.nq = nq; a.hord = hord; a.cap = cap;
-  const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
-  a.H = std::min(nj, 90);
+  a.H = std::min(nj, c.fused_h > 0 ? c.fused_h : 90);
   a.nstrip = (ni + FT_BX - 1) / FT_BX;
   a.nvblk = (nj + a.H - 1) / a.H;
   a.nsplit = nspl;
-  a.q_in = q_in; a.q_out = q_out; a.q_tmp = q_tmp;
+  a.q_in = q_in;
+  a.q_out = fast_remap ? q_tmp : q_out;
+  a.q_tmp = fast_remap ? q_out : q_tmp;
   a.cx = cx; a.cy = cy; a.mfx = mfx; a.mfy = mfy; a.dp1 = dp1;
   const GridDev<T> &g = grid_of<T>(c);
   a.area = g.area; a.rarea = g.rarea; a.dxa = g.dxa; a.dya = g.dya; a.dx = g.dx; a.dy = g.dy;
@@ -95,7 +111,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     if constexpr (std::is_same<T, double>::value)
       rc = launch_fused_f64(a, nblk, s);
     else
-      rc = launch_fused_f32(a, nblk, s);
+      rc = launch_fused_f32(a, nblk, c.fused_cfg, s);
     if (rc) return rc;
   }
   if (nsplit_cap > 0) {
@@ -105,10 +121,22 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   if (c.timing) FV3_HIP(hipEventRecord(c.ev[1], s));
   // remap: source thickness is dp1, target lands in delp_out
   FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
-  std::vector<int> kt(nq, kord_tr);
+  if (fast_remap && pt)
+    FV3_HIP(hipMemcpyAsync(pt_src, pt, sizeof(T) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
   if (c.timing) FV3_HIP(hipEventRecord(c.ev[2], s));
-  if (remap_tracers_plain<T>(c, npz, nq, ak, bk, ptop, delp_out, q_out, pt, kt.data(), kord_tm, t_min, 0))
-    return 1;
+  if constexpr (std::is_same<T, float>::value) {
+    if (fast_remap) {
+      if (remap_fast_f32(c, npz, nq, ak, bk, ptop, dp1, delp_out, q_tmp, q_out, pt_src, pt, kord_tr,
+                         kord_tm, t_min, cols))
+        return 1;
+    }
+  }
+  if (!fast_remap) {
+    std::vector<int> kt(nq, kord_tr);
+    if (remap_tracers_plain<T>(c, npz, nq, ak, bk, ptop, delp_out, q_out, pt, kt.data(), kord_tm,
+                               t_min, 0))
+      return 1;
+  }
   if (c.timing) {
     FV3_HIP(hipEventRecord(c.ev[3], s));
     c.last_passes = cap;

@@ -3,24 +3,28 @@
 //
 // Mapping.  A wavefront owns a strip of 64 columns (58 interior columns plus
 // the 3-cell halo on each side) and streams down the rows of a block of H
-// interior rows.  At each input row r it
+// interior rows, carrying NT tracers of one level at once.  At each input
+// row r it
 //   - slides the y-direction PPM window (registers only) to get fy2 at face
 //     r-2 and the transverse-updated q_i of row r-3,
-//   - sweeps row r-3 of q_i in x across lanes (shuffles) for fx,
+//   - sweeps row r-3 of q_i in x across lanes (DPP wave shifts) for fx,
 //   - sweeps row r of q in x for fx2 and forms q_j of row r,
 //   - slides the y-direction window over q_j for fy at face r-2,
 //   - averages the fluxes and writes the flux-form update of row r-3.
-// Nothing but q enters or leaves HBM per tracer; the winds, mass fluxes and
-// thicknesses of the level are shared by the waves of a workgroup (each wave
-// one tracer, the waves kept in step by a barrier per row, so the level data
-// is fetched once per workgroup).  The 1/nsplit scaling of cx, cy, mfx, mfy
-// and the dp1 <- dp2 chain of sub-cycling are applied in flight.
+// Nothing but q enters or leaves HBM per tracer.  The winds, mass fluxes and
+// thicknesses of the level are read once per row for the NT tracers of a
+// wave, and the waves of a workgroup (all on the same level and strip) are
+// kept in step by a barrier per row so those reads hit L1.  The 1/nsplit
+// scaling of cx, cy, mfx, mfy and the dp1 <- dp2 chain of sub-cycling are
+// applied in flight.
 //
-// Every expression keeps the upstream evaluation order: the T = double
-// instantiation, built with contraction off, reproduces the CPU FP64
-// evaluation; T = float is the bulk FP32 form (pressure thickness in FP64).
+// The T = double instantiation, built with contraction off, keeps the
+// upstream evaluation order and reproduces the CPU FP64 evaluation; T = float
+// is the bulk FP32 form (reciprocal multiplies, contraction on).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "fv3_device.h"
 
@@ -50,20 +54,52 @@ struct FusedArgs {
   long long ms2;                   // metric slab (isd:ied, jsd:jed)
 };
 
+// ---- cross-lane shifts: value of lane l-1 / l+1 (DPP wave_shr:1 / wave_shl:1)
+__device__ __forceinline__ int dpp_from_left(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_from_right(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+
 template <typename T>
-__device__ __forceinline__ T shup(T v, int d) {
-  return __shfl_up(v, d, FT_LANES);
+__device__ __forceinline__ T from_left(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(dpp_from_left(__float_as_int(v)));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_from_left((int)(b & 0xffffffffll)), hi = dpp_from_left((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
 }
 template <typename T>
-__device__ __forceinline__ T shdn(T v, int d) {
-  return __shfl_down(v, d, FT_LANES);
+__device__ __forceinline__ T from_right(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(dpp_from_right(__float_as_int(v)));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_from_right((int)(b & 0xffffffffll)), hi = dpp_from_right((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+
+// min/max as single instructions (max3/min3 fused); equal in value to the
+// upstream MIN/MAX on every finite input
+template <typename T> __device__ __forceinline__ T fmx(T a, T b) { return fmax(a, b); }
+template <typename T> __device__ __forceinline__ T fmn(T a, T b) { return fmin(a, b); }
+template <typename T> __device__ __forceinline__ T fmx3(T a, T b, T c) { return fmax(fmax(a, b), c); }
+template <typename T> __device__ __forceinline__ T fmn3(T a, T b, T c) { return fmin(fmin(a, b), c); }
+
+// x / y: exact for FP64, reciprocal multiply for FP32
+template <typename T>
+__device__ __forceinline__ T divide(T x, T y) {
+  if constexpr (sizeof(T) == 4)
+    return x * __builtin_amdgcn_rcpf(y);
+  else
+    return x / y;
 }
 
 // Lin's monotone slope from three cells
 template <typename T>
 __device__ __forceinline__ T dm3(T qm, T q0, T qp) {
   T xt = T(0.25) * (qp - qm);
-  return fsign(mn3(fabs(xt), mx3(qm, q0, qp) - q0, q0 - mn3(qm, q0, qp)), xt);
+  return fsign(fmn3(fabs(xt), fmx3(qm, q0, qp) - q0, q0 - fmn3(qm, q0, qp)), xt);
 }
 // face value between cells m-1 and m
 template <typename T>
@@ -74,8 +110,8 @@ __device__ __forceinline__ T al2(T qm, T q0, T dmm, T dm0) {
 template <typename T>
 __device__ __forceinline__ T edge_face(T a, T b, T c, T e, T da, T db, T dc, T de) {
   T xt = T(0.5) * (((T(2) * db + da) * b - db * a) / (da + db) + ((T(2) * dc + de) * c - dc * e) / (dc + de));
-  xt = mx2(xt, mn4(a, b, c, e));
-  xt = mn2(xt, mx4(a, b, c, e));
+  xt = fmx(xt, fmn(fmn3(a, b, c), e));
+  xt = fmn(xt, fmx(fmx3(a, b, c), e));
   return xt;
 }
 
@@ -88,46 +124,49 @@ __device__ __forceinline__ int edge_role(int m, int is, int ie, int npx, bool ed
   return 0;
 }
 
-// (bl, br) of one cell from its neighbourhood (xppm/yppm cell logic).
-// d0..d4: spacings of cells m-2..m+2 (read only for edge roles).
+// (bl, br) of a cell with a cube-edge role (xppm/yppm edge code).
+// d0..d4: spacings of cells m-2..m+2.
+template <typename T>
+__device__ __forceinline__ void edge_coef(T qm2, T qm1, T q0, T qp1, T qp2, T dmm1, T dm0, T dmp1,
+                                       T al0, T alp1, int role, T d0, T d1, T d2, T d3, T d4,
+                                       T &bl, T &br) {
+  switch (role) {
+    case 1:
+      bl = K<T>::s14 * dmm1 + K<T>::s11 * (qm1 - q0);
+      br = edge_face(qm1, q0, qp1, qp2, d1, d2, d3, d4) - q0;
+      break;
+    case 2:
+      bl = edge_face(qm2, qm1, q0, qp1, d0, d1, d2, d3) - q0;
+      br = K<T>::s15 * q0 + K<T>::s11 * qp1 - K<T>::s14 * dmp1 - q0;
+      break;
+    case 3:
+      bl = K<T>::s15 * qm1 + K<T>::s11 * q0 - K<T>::s14 * dm0 - q0;
+      br = alp1 - q0;
+      break;
+    case 4:
+      bl = al0 - q0;
+      br = K<T>::s15 * qp1 + K<T>::s11 * q0 + K<T>::s14 * dm0 - q0;
+      break;
+    case 5:
+      bl = K<T>::s15 * q0 + K<T>::s11 * qm1 + K<T>::s14 * dmm1 - q0;
+      br = edge_face(qm1, q0, qp1, qp2, d1, d2, d3, d4) - q0;
+      break;
+    default:
+      bl = edge_face(qm2, qm1, q0, qp1, d0, d1, d2, d3) - q0;
+      br = K<T>::s11 * (qp1 - q0) - K<T>::s14 * dmp1;
+      break;
+  }
+  pert_ppm1(q0, bl, br, 1);
+}
+
+// (bl, br) of an interior cell for the monotone orders
 template <typename T>
 __device__ __forceinline__ void cell_coef(T qm2, T qm1, T q0, T qp1, T qp2, T dmm1, T dm0, T dmp1,
-                                          T al0, T alp1, int role, int iord, T d0, T d1, T d2,
-                                          T d3, T d4, T &bl, T &br) {
-  if (role != 0) {
-    switch (role) {
-      case 1:
-        bl = K<T>::s14 * dmm1 + K<T>::s11 * (qm1 - q0);
-        br = edge_face(qm1, q0, qp1, qp2, d1, d2, d3, d4) - q0;
-        break;
-      case 2:
-        bl = edge_face(qm2, qm1, q0, qp1, d0, d1, d2, d3) - q0;
-        br = K<T>::s15 * q0 + K<T>::s11 * qp1 - K<T>::s14 * dmp1 - q0;
-        break;
-      case 3:
-        bl = K<T>::s15 * qm1 + K<T>::s11 * q0 - K<T>::s14 * dm0 - q0;
-        br = alp1 - q0;
-        break;
-      case 4:
-        bl = al0 - q0;
-        br = K<T>::s15 * qp1 + K<T>::s11 * q0 + K<T>::s14 * dm0 - q0;
-        break;
-      case 5:
-        bl = K<T>::s15 * q0 + K<T>::s11 * qm1 + K<T>::s14 * dmm1 - q0;
-        br = edge_face(qm1, q0, qp1, qp2, d1, d2, d3, d4) - q0;
-        break;
-      default:
-        bl = edge_face(qm2, qm1, q0, qp1, d0, d1, d2, d3) - q0;
-        br = K<T>::s11 * (qp1 - q0) - K<T>::s14 * dmp1;
-        break;
-    }
-    pert_ppm1(q0, bl, br, 1);
-    return;
-  }
+                                          T al0, T alp1, int iord, T &bl, T &br) {
   if (iord == 8 || iord == 11) {
     T xt = iord == 8 ? T(2) * dm0 : K<T>::ppm_fac * dm0;
-    bl = -fsign(mn2(fabs(xt), fabs(al0 - q0)), xt);
-    br = fsign(mn2(fabs(xt), fabs(alp1 - q0)), xt);
+    bl = -fsign(fmn(fabs(xt), fabs(al0 - q0)), xt);
+    br = fsign(fmn(fabs(xt), fabs(alp1 - q0)), xt);
   } else {
     bl = al0 - q0;
     br = alp1 - q0;
@@ -138,10 +177,10 @@ __device__ __forceinline__ void cell_coef(T qm2, T qm1, T q0, T qp1, T qp2, T dm
       T dqm2 = T(2) * (qm1 - qm2), dqm1 = T(2) * (q0 - qm1), dq0 = T(2) * (qp1 - q0), dqp1 = T(2) * (qp2 - qp1);
       T pmp_2 = dqm1;
       T lac_2 = pmp_2 - T(0.75) * dqm2;
-      br = mn2(mx3(T(0), pmp_2, lac_2), mx2(br, mn3(T(0), pmp_2, lac_2)));
+      br = fmn(fmx3(T(0), pmp_2, lac_2), fmx(br, fmn3(T(0), pmp_2, lac_2)));
       T pmp_1 = -dq0;
       T lac_1 = pmp_1 + T(0.75) * dqp1;
-      bl = mn2(mx3(T(0), pmp_1, lac_1), mx2(bl, mn3(T(0), pmp_1, lac_1)));
+      bl = fmn(fmx3(T(0), pmp_1, lac_1), fmx(bl, fmn3(T(0), pmp_1, lac_1)));
     }
     if (iord == 9 || iord == 13) pert_ppm1(q0, bl, br, 0);
   }
@@ -159,68 +198,54 @@ __device__ __forceinline__ T face_flux(T c, T qu, T blu, T bru) {
 template <typename T>
 __device__ __forceinline__ T xsweep(T q, T c, int i, int is, int ie, int npx, bool edge,
                                     bool edge_strip, int iord, T dxl) {
-  const T qm1 = shup(q, 1), qp1 = shdn(q, 1);
+  const T qm1 = from_left(q), qp1 = from_right(q);
   const T dm = dm3(qm1, q, qp1);
-  const T dmm1 = shup(dm, 1), dmp1 = shdn(dm, 1);
+  const T dmm1 = from_left(dm), dmp1 = from_right(dm);
   const T al = al2(qm1, q, dmm1, dm);
-  const T alp1 = shdn(al, 1);
+  const T alp1 = from_right(al);
   T qm2 = q, qp2 = q;
   if (iord != 8 || edge_strip) {
-    qm2 = shup(q, 2);
-    qp2 = shdn(q, 2);
-  }
-  T d0 = dxl, d1 = dxl, d3 = dxl, d4 = dxl;
-  int role = 0;
-  if (edge_strip) {
-    d0 = shup(dxl, 2);
-    d1 = shup(dxl, 1);
-    d3 = shdn(dxl, 1);
-    d4 = shdn(dxl, 2);
-    role = edge_role(i, is, ie, npx, edge);
+    qm2 = from_left(qm1);
+    qp2 = from_right(qp1);
   }
   T bl, br;
-  cell_coef(qm2, qm1, q, qp1, qp2, dmm1, dm, dmp1, al, alp1, role, iord, d0, d1, dxl, d3, d4, bl, br);
-  const T blm = shup(bl, 1), brm = shup(br, 1);
+  cell_coef(qm2, qm1, q, qp1, qp2, dmm1, dm, dmp1, al, alp1, iord, bl, br);
+  if (edge_strip) {
+    const int role = edge_role(i, is, ie, npx, edge);
+    const T d1 = from_left(dxl), d3 = from_right(dxl);
+    const T d0 = from_left(d1), d4 = from_right(d3);
+    if (role != 0) edge_coef(qm2, qm1, q, qp1, qp2, dmm1, dm, dmp1, al, alp1, role, d0, d1, dxl, d3, d4, bl, br);
+  }
+  const T blm = from_left(bl), brm = from_left(br);
   return c > T(0) ? face_flux(c, qm1, blm, brm) : face_flux(c, q, bl, br);
 }
 
 // Sliding y-direction PPM over a column: fed one cell per row, yields the
-// flux at face r-2 once rows r-5..r are in.  Histories are indexed by the
+// flux at face r-2 once rows r-4..r are in.  Histories are indexed by the
 // row modulo 6 so that an unrolled loop keeps them in fixed registers.
 template <typename T>
 struct YPipe {
-  T w[6];   // q of rows r-5..r, slot row%6
-  T dm[6];  // dm of cells, slot cell%6
-  T al[6];  // face values, slot face%6
+  T w[6];   // q of rows, slot row%6
+  T dm[6];  // dm of cells
+  T al[6];  // face values
   T bl[6], br[6];
 };
 
-// one step at input row r; U is the slot of row r (the step index modulo 6).
+// One step at input row r; U is the slot of row r.  role/dd: the cube-edge
+// role of cell r-2 and the spacings of cells r-4..r (uniform over the wave).
 template <typename T, int U>
-__device__ __forceinline__ T ypipe_step(YPipe<T> &p, T qnew, int r, T c, int js, int je, int npy,
-                                        bool edge, int iord, const T *dcol, long long dstride,
-                                        int jsd, int jed) {
+__device__ __forceinline__ T ypipe_step(YPipe<T> &p, T qnew, T c, int iord, int role, T d0, T d1,
+                                        T d2, T d3, T d4) {
   constexpr int s0 = U % 6, s1 = (U + 5) % 6, s2 = (U + 4) % 6, s3 = (U + 3) % 6,
                 s4 = (U + 2) % 6;  // rows r, r-1, r-2, r-3, r-4
   p.w[s0] = qnew;
-  // dm of cell r-1, face value al at face r-1 (between cells r-2 and r-1)
   p.dm[s1] = dm3(p.w[s2], p.w[s1], p.w[s0]);
   p.al[s1] = al2(p.w[s2], p.w[s1], p.dm[s2], p.dm[s1]);
-  // coefficients of cell m = r-2
-  const int m = r - 2;
-  const int role = edge_role(m, js, je, npy, edge);
-  T d0 = T(0), d1 = T(0), d2 = T(0), d3 = T(0), d4 = T(0);
-  if (role != 0) {  // uniform across the wave (all lanes share the row)
-    auto dd = [&](int jj) { jj = jj < jsd ? jsd : (jj > jed ? jed : jj); return dcol[(long long)(jj - jsd) * dstride]; };
-    d0 = dd(m - 2);
-    d1 = dd(m - 1);
-    d2 = dd(m);
-    d3 = dd(m + 1);
-    d4 = dd(m + 2);
-  }
   cell_coef(p.w[s4], p.w[s3], p.w[s2], p.w[s1], p.w[s0], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2],
-            p.al[s1], role, iord, d0, d1, d2, d3, d4, p.bl[s2], p.br[s2]);
-  // face r-2: upwind cell r-3 (c > 0) or r-2
+            p.al[s1], iord, p.bl[s2], p.br[s2]);
+  if (role != 0)
+    edge_coef(p.w[s4], p.w[s3], p.w[s2], p.w[s1], p.w[s0], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2],
+              p.al[s1], role, d0, d1, d2, d3, d4, p.bl[s2], p.br[s2]);
   return c > T(0) ? face_flux(c, p.w[s3], p.bl[s3], p.br[s3]) : face_flux(c, p.w[s2], p.bl[s2], p.br[s2]);
 }
 

@@ -71,6 +71,10 @@ struct Ctx {
   std::mutex mu;
   // optional phase timing of fv3_step_dev (hipEvents on the step's stream)
   bool timing = false;
+  // fused-kernel launch shape: configuration id and interior rows per block
+  int fused_cfg = 0, fused_h = 0;
+  // FP32 step remap: 0 column-shared fast kernel, 1 one thread per column
+  int remap_mode = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int last_passes = 0;
   bool ev_valid = false;

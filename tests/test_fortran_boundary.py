@@ -1,0 +1,82 @@
+"""The Fortran drop-in boundary: fv_tracer2d_mod / fv_mapz_mod / tp_core_mod
+with the dycore's argument lists, bound to libfv3atm_amd.so through
+ISO_C_BINDING (fv3atm_amd/fortran).  A Fortran program (fv3_harness) calls
+tracer_2d_1L and then the tracer/pt remap exactly as a dycore caller would;
+on the GPU its results must equal the FP64 oracle to 1e-12 per cell.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fv3atm_amd import cases
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(ROOT, "fv3atm_amd", "fv3_harness")
+NG = 3
+
+needs_harness = pytest.mark.skipif(not os.path.exists(HARNESS),
+                                   reason="Fortran harness not built (amdflang absent)")
+
+
+@needs_harness
+def test_fortran_host_checks():
+    # status codes and the last-error text cross the boundary (no device work)
+    r = subprocess.run([HARNESS, "--host"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "host checks passed" in r.stdout
+
+
+def write_input(path, case, fill=0, trdm=0.0, lim_fac=1.0, t_min=184.0):
+    g = case.grids[0]
+    hdr = np.array([case.n, case.npz, case.nq, case.hord, case.kord_tr, case.kord_tm, fill,
+                    int(g.sw_corner), int(g.se_corner), int(g.ne_corner), int(g.nw_corner),
+                    g.grid_type], dtype="<i4")
+    rhdr = np.array([case.ptop, t_min, case.dt, trdm, lim_fac, g.da_min], dtype="<f8")
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(rhdr.tobytes())
+        for a in (g.area, g.rarea, g.dxa, g.dya, g.dx, g.dy, g.sin_sg, g.del6_u, g.del6_v,
+                  case.q, case.dp1, case.mfx, case.mfy, case.cx, case.cy, case.ak, case.bk,
+                  case.pt):
+            f.write(np.asarray(a, dtype="<f8").tobytes(order="F"))
+
+
+def read_output(path, case):
+    nd = case.n + 2 * NG
+    raw = open(path, "rb").read()
+    sizes = [nd * nd * case.npz * case.nq, nd * nd * case.npz, nd * nd * case.npz,
+             nd * nd * case.npz]
+    out, off = [], 0
+    for n in sizes:
+        out.append(np.frombuffer(raw, dtype="<f8", count=n, offset=off))
+        off += 8 * n
+    nsplit = np.frombuffer(raw, dtype="<i4", count=case.npz, offset=off)
+    q = out[0].reshape((nd, nd, case.npz, case.nq), order="F")
+    shp = (nd, nd, case.npz)
+    return dict(q=q, dp1=out[1].reshape(shp, order="F"), delp=out[2].reshape(shp, order="F"),
+                pt=out[3].reshape(shp, order="F"), nsplit=nsplit)
+
+
+@needs_harness
+@pytest.mark.gpu
+def test_fortran_step_matches_oracle(gpu_ready, tmp_path):
+    from oracle import oracle
+    from tests._util import assert_close, interior
+    case = cases.make_case("C20L9Q6jet", n=20, npz=9, nq=6, jet=True, courant=2.3, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=12, with_pt=True)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_input(fin, case)
+    print("fortran harness C20L9Q6jet", flush=True)
+    r = subprocess.run([HARNESS, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_output(fout, case)
+    adv = oracle.tracer_2d(case, "1L")
+    q, delp, pt = oracle.remap_tracers(case, adv["q"], case.dp1, case.pt)
+    np.testing.assert_array_equal(got["nsplit"], adv["nsplit"])
+    assert got["nsplit"].max() >= 2
+    assert_close(interior(got["dp1"]), interior(adv["dp1"]), 1e-12, "dp1")
+    assert_close(interior(got["q"]), interior(q), 1e-12, "q", floor=1e-300)
+    assert_close(interior(got["delp"]), interior(delp), 1e-12, "delp")
+    assert_close(interior(got["pt"]), interior(pt), 1e-12, "pt")
