@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="first run a 1 GiB copy kernel (reference for HBM byte counters)")
     ap.add_argument("--launch", type=int, default=-1,
                     help="fused launch configuration; -1 tunes over all of them first")
     args = ap.parse_args()
@@ -129,6 +131,13 @@ def main():
     from fv3atm_amd import cases
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.calib:  # reads 1 GiB and writes 1 GiB in one elementwise kernel
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
+        log("calibration copy: 1 GiB read + 1 GiB write (vectorized_elementwise mul)")
     t0 = time.perf_counter()
     case = cases.config(2, n=args.n, npz=args.npz, nq=args.nq, build_q=False)
     nspl = nsplit_of(case)

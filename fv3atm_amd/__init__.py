@@ -158,6 +158,38 @@ def step_dev(ctx: Context, form, nq, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx
                                   stream), "fv3_step_dev")
 
 
+def step_tl_dev(ctx: Context, per_level, q_split, nq, q0, dq0, q1, dq1, dp1, delp_out, cx, cy, mfx,
+                mfy, ak, bk, ptop, hord=8, kord_tr=9, stream=None):
+    """Tangent-linear of the FP64 advection + tracer remap step (frozen
+    branches) on torch device tensors: (q0, dq0) -> (q1, dq1)."""
+    check(_ffi.lib().fv3_step_tl_dev(ctx.handle, int(per_level), q_split, nq, ptr(q0), ptr(dq0),
+                                     ptr(q1), ptr(dq1), ptr(dp1), ptr(delp_out), ptr(cx), ptr(cy),
+                                     ptr(mfx), ptr(mfy), ptr(ak), ptr(bk), ptop, hord, kord_tr,
+                                     stream), "fv3_step_tl_dev")
+
+
+def step_ad_dev(ctx: Context, per_level, q_split, nq, q0, qbar1, qbar0, dp1, cx, cy, mfx, mfy, ak,
+                bk, ptop, hord=8, kord_tr=9, stream=None):
+    """Adjoint of the same step: qbar1 (adjoint of the output) -> qbar0."""
+    check(_ffi.lib().fv3_step_ad_dev(ctx.handle, int(per_level), q_split, nq, ptr(q0), ptr(qbar1),
+                                     ptr(qbar0), ptr(dp1), ptr(cx), ptr(cy), ptr(mfx), ptr(mfy),
+                                     ptr(ak), ptr(bk), ptop, hord, kord_tr, stream),
+          "fv3_step_ad_dev")
+
+
+def set_graph_safe(ctx: Context, on: bool = True):
+    """Make fv3_step_dev capturable in a hipGraph (nsplit_cap > 0 required)."""
+    check(_ffi.lib().fv3_ctx_set_graph_safe(ctx.handle, int(on)), "fv3_ctx_set_graph_safe")
+
+
+def step_status(ctx: Context) -> int:
+    """1 when a level of the last graph-safe step needed more sub-cycles than
+    nsplit_cap (its results are not valid), else 0."""
+    v = ctypes.c_int()
+    check(_ffi.lib().fv3_step_status(ctx.handle, ctypes.byref(v)), "fv3_step_status")
+    return v.value
+
+
 def set_timing(ctx: Context, on: bool):
     """Record device times of the transport passes and the remap of each
     fused step (hipEvents on the step's stream)."""

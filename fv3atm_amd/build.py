@@ -30,6 +30,7 @@ HIP_SOURCES = {
     "fused.hip": ["-ffp-contract=off"],
     "fused32.hip": ["-ffp-contract=fast"],
     "mapz_fast.hip": ["-ffp-contract=fast"],
+    "tlad.hip": ["-ffp-contract=off"],
 }
 
 

@@ -240,6 +240,7 @@ int fv3_ctx_destroy(fv3_ctx *h) {
   (void)hipFree(c.halo_src);
   (void)hipFree(c.scratch);
   (void)hipFree(c.stage);
+  (void)hipFree(c.step_flag);
   for (auto &e : c.ev)
     if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -693,6 +694,75 @@ int fv3_ctx_set_launch(fv3_ctx *h, int cfg, int rows, int remap) {
   c.fused_cfg = cfg;
   c.fused_h = rows;
   c.remap_mode = remap;
+  return 0;
+}
+
+// ---- tangent-linear / adjoint of the FP64 step --------------------------------
+static int tlad_args(fv3_ctx *h, int nq, const void *a, const void *b, const void *cc, const double *dp1,
+                     const double *cx, const double *cy, const double *mfx, const double *mfy,
+                     const double *ak, const double *bk, int kord_tr) {
+  if (!h) return set_error("NULL context");
+  if (!h->c.grid_set) return set_error("grid metrics not uploaded (fv3_ctx_set_grid)");
+  if (!a || !b || !cc || !dp1 || !cx || !cy || !mfx || !mfy) return set_error("NULL argument");
+  if (kord_tr && (!ak || !bk)) return set_error("NULL ak/bk with the remap enabled");
+  if (nq < 1) return set_error("nq=%d < 1", nq);
+  return 0;
+}
+
+int fv3_step_tl_dev(fv3_ctx *h, int per_level, int q_split, int nq, const double *q0,
+                    const double *dq0, double *q1, double *dq1, const double *dp1,
+                    double *delp_out, const double *cx, const double *cy, const double *mfx,
+                    const double *mfy, const double *ak, const double *bk, double ptop, int hord,
+                    int kord_tr, void *stream) {
+  if (tlad_args(h, nq, q0, dq0, q1, dp1, cx, cy, mfx, mfy, ak, bk, kord_tr)) return 1;
+  if (!dq1) return set_error("NULL argument");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  hipStream_t saved = c.stream;
+  if (stream) c.stream = (hipStream_t)stream;
+  const int rc = step_tl(c, per_level != 0, q_split, nq, q0, dq0, q1, dq1, dp1, delp_out, cx, cy, mfx, mfy,
+                         ak, bk, ptop, hord, kord_tr);
+  c.stream = saved;
+  return rc;
+}
+
+int fv3_step_ad_dev(fv3_ctx *h, int per_level, int q_split, int nq, const double *q0,
+                    const double *qbar1, double *qbar0, const double *dp1, const double *cx,
+                    const double *cy, const double *mfx, const double *mfy, const double *ak,
+                    const double *bk, double ptop, int hord, int kord_tr, void *stream) {
+  if (tlad_args(h, nq, q0, qbar1, qbar0, dp1, cx, cy, mfx, mfy, ak, bk, kord_tr)) return 1;
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  hipStream_t saved = c.stream;
+  if (stream) c.stream = (hipStream_t)stream;
+  const int rc = step_ad(c, per_level != 0, q_split, nq, q0, qbar1, qbar0, dp1, cx, cy, mfx, mfy, ak, bk,
+                         ptop, hord, kord_tr);
+  c.stream = saved;
+  return rc;
+}
+
+// ---- graph-safe steps --------------------------------------------------------
+int fv3_ctx_set_graph_safe(fv3_ctx *h, int on) {
+  if (!h) return set_error("NULL context");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (on && !c.step_flag) {
+    FV3_HIP(hipMalloc(&c.step_flag, sizeof(int)));
+    FV3_HIP(hipMemset(c.step_flag, 0, sizeof(int)));
+  }
+  c.graph_safe = on != 0;
+  return 0;
+}
+
+int fv3_step_status(fv3_ctx *h, int *nsplit_overflow) {
+  if (!h || !nsplit_overflow) return set_error("fv3_step_status: NULL argument");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.step_flag) return set_error("fv3_step_status: graph-safe mode was never enabled");
+  int v = 0;
+  FV3_HIP(hipMemcpyAsync(&v, c.step_flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  FV3_HIP(hipStreamSynchronize(c.stream));
+  *nsplit_overflow = v;
   return 0;
 }
 

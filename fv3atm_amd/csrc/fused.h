@@ -22,4 +22,14 @@ int fused_step(Ctx &c, bool fp32, int nq, const void *q_in, void *q_out, void *p
                const void *mfx, const void *mfy, const double *ak, const double *bk, double ptop,
                int hord, int kord_tr, int kord_tm, double t_min, int nsplit_cap);
 
+// tangent-linear / adjoint of the FP64 advection + tracer remap step
+int step_tl(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const double *dq0,
+            double *q1, double *dq1, const double *dp1, double *delp_out, const double *cx,
+            const double *cy, const double *mfx, const double *mfy, const double *ak,
+            const double *bk, double ptop, int hord, int kord_tr);
+int step_ad(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const double *qbar1,
+            double *qbar0, const double *dp1, const double *cx, const double *cy,
+            const double *mfx, const double *mfy, const double *ak, const double *bk,
+            double ptop, int hord, int kord_tr);
+
 }  // namespace fv3

@@ -35,6 +35,8 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   hipStream_t s = c.stream;
   if (c.ntiles != 1) return set_error("the fused forms take a single-tile context");
   if (hord < 7 || hord > 13) return set_error("hord=%d: only the monotone orders 7..13 are implemented", hord);
+  if (c.graph_safe && nsplit_cap <= 0)
+    return set_error("graph-safe steps need nsplit_cap > 0 (no device-to-host read)");
   // scratch: cmax, frac, nsplit, flag, then q_tmp
   size_t off = 0;
   auto take = [&](size_t b) {
@@ -48,7 +50,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   char *base = (char *)ctx_stage(c, head);
   if (!base) return 1;
   T *cmax = (T *)(base + o_cmax), *frac = (T *)(base + o_frac);
-  int *nspl = (int *)(base + o_ns), *flag = (int *)(base + o_flag);
+  int *nspl = (int *)(base + o_ns), *flag = c.graph_safe ? c.step_flag : (int *)(base + o_flag);
   if (device_nsplit<T>(c, cx, cy, true, 0, cmax, nspl, frac)) return 1;
   int cap = nsplit_cap;
   if (cap <= 0) {
@@ -104,7 +106,8 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     a.gym = a.gyp + (long long)d.nid * (d.njd + 1);
   }
   const int nblk = a.nstrip * a.nvblk;
-  if (c.timing) FV3_HIP(hipEventRecord(c.ev[0], s));
+  const bool timing = c.timing && !c.graph_safe;
+  if (timing) FV3_HIP(hipEventRecord(c.ev[0], s));
   for (int it = 1; it <= cap; it++) {
     a.it = it;
     int rc;
@@ -118,12 +121,12 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     hipLaunchKernelGGL(k_nsplit_check, dim3(1), dim3(1), 0, s, nspl, npz, cap, flag);
     FV3_HIP(hipGetLastError());
   }
-  if (c.timing) FV3_HIP(hipEventRecord(c.ev[1], s));
+  if (timing) FV3_HIP(hipEventRecord(c.ev[1], s));
   // remap: source thickness is dp1, target lands in delp_out
   FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
   if (fast_remap && pt)
     FV3_HIP(hipMemcpyAsync(pt_src, pt, sizeof(T) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
-  if (c.timing) FV3_HIP(hipEventRecord(c.ev[2], s));
+  if (timing) FV3_HIP(hipEventRecord(c.ev[2], s));
   if constexpr (std::is_same<T, float>::value) {
     if (fast_remap) {
       if (remap_fast_f32(c, npz, nq, ak, bk, ptop, dp1, delp_out, q_tmp, q_out, pt_src, pt, kord_tr,
@@ -137,12 +140,12 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
                                t_min, 0))
       return 1;
   }
-  if (c.timing) {
+  if (timing) {
     FV3_HIP(hipEventRecord(c.ev[3], s));
     c.last_passes = cap;
     c.ev_valid = true;
   }
-  if (nsplit_cap > 0) {
+  if (nsplit_cap > 0 && !c.graph_safe) {
     int hf = 0;
     FV3_HIP(hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, s));
     FV3_HIP(hipStreamSynchronize(s));

@@ -75,6 +75,10 @@ struct Ctx {
   int fused_cfg = 0, fused_h = 0;
   // FP32 step remap: 0 column-shared fast kernel, 1 one thread per column
   int remap_mode = 0;
+  // graph-safe steps: no host synchronisation, no memset, no allocation; the
+  // nsplit_cap check lands in step_flag for fv3_step_status
+  bool graph_safe = false;
+  int *step_flag = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int last_passes = 0;
   bool ev_valid = false;

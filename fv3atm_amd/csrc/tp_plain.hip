@@ -117,12 +117,19 @@ __global__ void k_nsplit(const T *cmax, int npz, int per_level, int q_split, int
   }
 }
 
+// zero fill by a kernel: unlike a memset node it is safe inside a captured graph
+template <typename T>
+__global__ void k_zero(T *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = T(0);
+}
+
 template <typename T>
 int device_nsplit(Ctx &c, const T *cx, const T *cy, bool per_level, int q_split, T *cmax,
                   int *nsplit, T *frac) {
   const Dims d = c.d;
   const int npz = c.npz, nt = c.ntiles;
-  FV3_HIP(hipMemsetAsync(cmax, 0, sizeof(T) * npz, c.stream));
+  hipLaunchKernelGGL(k_zero<T>, dim3(1), dim3(256), 0, c.stream, cmax, npz);
+  FV3_HIP(hipGetLastError());
   dim3 b(256), gr((unsigned)std::min(64, (d.ie - d.is + 1) * (d.je - d.js + 1) / 256 + 1), 1,
                   (unsigned)(npz * nt));
   hipLaunchKernelGGL(k_cmax<T>, gr, b, 0, c.stream, cx, cy, grid_of<T>(c).sin_sg, d, npz,

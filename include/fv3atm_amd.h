@@ -143,6 +143,38 @@ int fv3_ctx_set_timing(fv3_ctx *ctx, int on);
  * the two remap kernels agree to FP32 rounding. */
 int fv3_ctx_set_launch(fv3_ctx *ctx, int cfg, int rows, int remap);
 int fv3_step_timing(fv3_ctx *ctx, double *transport_ms, double *remap_ms, int *passes);
+/* Graph-safe steps (no counterpart upstream): with on != 0, fv3_step_dev
+ * makes no host synchronisation, allocation or memset once its scratch is
+ * sized (run one step before capturing), so a caller can capture it in a
+ * hipGraph.  It then requires nsplit_cap > 0, skips the phase timing, and
+ * leaves the "a level needs more than nsplit_cap sub-cycles" check on the
+ * device: fv3_step_status returns it for the last step (1 = overflow, whose
+ * results are not valid). */
+int fv3_ctx_set_graph_safe(fv3_ctx *ctx, int on);
+int fv3_step_status(fv3_ctx *ctx, int *nsplit_overflow);
+
+/* Tangent-linear and adjoint of the FP64 advection + tracer remap step: the
+ * derivatives of tracer_2d (per_level = 0; q_split > 0 forces nsplit) or
+ * tracer_2d_1L (per_level = 1) followed by the tracer part of
+ * Lagrangian_to_Eulerian (kord_tr; 0 leaves the remap out), with respect to
+ * the interior tracer values, every limiter branch, upwind choice and
+ * source-layer search frozen at the forward trajectory q0.  Winds, dp1 and
+ * the single tile's halo are fixed.  Device pointers, layouts as
+ * fv3_step_dev, all double.  The upstream dycore has no such routines; they
+ * serve variational and sensitivity work on this path.
+ * TL: (q0, dq0) -> (q1, dq1); q1 is the forward result, delp_out (may be
+ * NULL) the target thickness.  AD: qbar1 (adjoint of q1) -> qbar0 (adjoint of
+ * q0's interior; halo cells 0).  The adjoint recomputes each sub-cycle from
+ * checkpointed states. */
+int fv3_step_tl_dev(fv3_ctx *ctx, int per_level, int q_split, int nq, const double *q0,
+                    const double *dq0, double *q1, double *dq1, const double *dp1,
+                    double *delp_out, const double *cx, const double *cy, const double *mfx,
+                    const double *mfy, const double *ak, const double *bk, double ptop, int hord,
+                    int kord_tr, void *stream);
+int fv3_step_ad_dev(fv3_ctx *ctx, int per_level, int q_split, int nq, const double *q0,
+                    const double *qbar1, double *qbar0, const double *dp1, const double *cx,
+                    const double *cy, const double *mfx, const double *mfy, const double *ak,
+                    const double *bk, double ptop, int hord, int kord_tr, void *stream);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,32 @@
  * fastest, lower bounds carried explicitly (isd:ied, jsd:jed, 1:npz, 1:nq),
  * as at ccpp/data/CCPP_typedefs.F90:1580-1590 and
  * moving_nest/fv_moving_nest.F90:177-189.
+ *
+ * Parity status.  The reference holds no golden vectors, known-answer tests
+ * or fixtures for this path, and its upstream source is absent, so no
+ * function can be compared with upstream output.  Each function is pinned
+ * instead by what the mathematics fixes (tests/test_oracle_tp.py,
+ * tests/test_oracle_mapz.py):
+ *   xppm/yppm interior (hord 8-13): exact on linear data (all orders) and on
+ *     parabola means (hord 8); unit Courant takes the upwind cell; yppm is
+ *     xppm transposed; hord 8/11 obey the discrete max principle.
+ *   pert_ppm iv=1: the constrained parabola is monotone in the cell.
+ *   copy_corners: dir 1 and dir 2 agree under the diagonal swap; six-tile
+ *     constancy and mass conservation.
+ *   fv_tp_2d, tracer_2d_1L, tracer_2d: constant preservation, six-tile mass
+ *     conservation to 1e-13, the nsplit rule, forced q_split.
+ *   deln_flux: zero flux on constants, variance reduction on noise.
+ *   cs_profile: C1 spline when unlimited, exact on linear profiles.
+ *   map1_ppm, map_scalar, mapn_tracer, Lagrangian_to_Eulerian subset:
+ *     identity remap, column conservation, constancy, refine-then-coarsen,
+ *     iv=0 positivity, pressure bookkeeping.
+ *   fillz: conservation and non-negativity.
+ * Parity unpinned (no check can tell a faithful restatement from a
+ * plausible variant): the cube-edge one-sided coefficients of xppm/yppm
+ * beyond their linear exactness, the copy_corners orientation relative to
+ * upstream beyond its symmetry, the 1 - sin_sg(5) term of the nsplit rule,
+ * the hord 9/10/12/13 Huynh-constraint details, and the kord 9/10 extremum
+ * branches of cs_profile beyond conservation, positivity and exactness.
  */
 #ifndef FV3_ORACLE_H
 #define FV3_ORACLE_H
