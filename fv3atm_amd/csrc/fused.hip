@@ -10,9 +10,21 @@
 
 namespace fv3 {
 
+template <int HO>
+static void go64(const FusedArgs<double> &a, int nblk, hipStream_t s) {
+  const dim3 b(FT_LANES * 4);
+  const int nin = a.s_hi >= a.s_lo && a.v_hi >= a.v_lo ? (a.s_hi - a.s_lo + 1) * (a.v_hi - a.v_lo + 1) : 0;
+  if (nin > 0)
+    hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, false>), dim3((unsigned)nin, (unsigned)a.npz, 1), b, 0, s, a);
+  if (nin < nblk)
+    hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, true>), dim3((unsigned)nblk, (unsigned)a.npz, 1), b, 0, s, a);
+}
+
 int launch_fused_f64(const FusedArgs<double> &a, int nblk, hipStream_t s) {
-  dim3 g((unsigned)nblk, (unsigned)a.npz, 1), b(FT_LANES * 4);
-  hipLaunchKernelGGL((k_fused_tp<double, 4, 1>), g, b, 0, s, a);
+  if (a.hord == 8)
+    go64<8>(a, nblk, s);
+  else
+    go64<0>(a, nblk, s);
   FV3_HIP(hipGetLastError());
   return 0;
 }
@@ -91,6 +103,29 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   a.H = std::min(nj, c.fused_h > 0 ? c.fused_h : 90);
   a.nstrip = (ni + FT_BX - 1) / FT_BX;
   a.nvblk = (nj + a.H - 1) / a.H;
+  // interior blocks: every lane's column i0-3..i0+60 and every input row
+  // j0-3..j0+H+2 lie in the compute domain (no halo lane or row) and, on a
+  // cube face, every stencil cell is 3 or more cells inside the tile (no
+  // cube-edge role, no corner)
+  {
+    const bool edges = d.edge || (!d.bounded && d.corners != 0u);
+    auto strip_ok = [&](int st) {
+      const int i0 = d.is + st * FT_BX;
+      const int lo = i0 - NG, hi = i0 - NG + FT_LANES - 1;
+      return lo >= d.is && hi <= d.ie && (!edges || (lo >= 3 && hi <= d.npx - 3));
+    };
+    auto vb_ok = [&](int v) {
+      const int j0 = d.js + v * a.H;
+      return j0 - NG >= d.js && j0 + a.H + 2 <= d.je && (!edges || (j0 - 5 >= 3 && j0 + a.H <= d.npy - 3));
+    };
+    a.s_lo = 1; a.s_hi = 0; a.v_lo = 1; a.v_hi = 0;
+    int s0 = -1, s1 = -2, v0 = -1, v1 = -2;
+    for (int st = 0; st < a.nstrip; st++)
+      if (strip_ok(st)) { if (s0 < 0) s0 = st; s1 = st; }
+    for (int v = 0; v < a.nvblk; v++)
+      if (vb_ok(v)) { if (v0 < 0) v0 = v; v1 = v; }
+    if (s0 >= 0 && v0 >= 0) { a.s_lo = s0; a.s_hi = s1; a.v_lo = v0; a.v_hi = v1; }
+  }
   a.nsplit = nspl;
   a.q_in = q_in;
   a.q_out = fast_remap ? q_tmp : q_out;

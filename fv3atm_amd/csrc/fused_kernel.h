@@ -18,7 +18,7 @@ __device__ __forceinline__ void pass_buffers(const FusedArgs<T> &a, int nspl, co
 template <typename T>
 struct StripCtx {
   int i, ic, j0, nix, niy, lo_q, lo_fx, lo_mx;  // lane column; lane offsets in the row layouts
-  bool col_out, lane_in, edge_strip, corner_wg, scale;
+  bool col_out, lane_in, edge_strip, edge_rows, corner_wg, scale;
   T frac;
   const T *cxk, *cyk, *mfxk, *mfyk;
   const double *dp1k;
@@ -41,56 +41,143 @@ struct LvState {
   T yfx_prev, mfy_prev;
 };
 
-// One input row r = j0 - 3 + t of the streaming pass; U = t mod 6.
-template <typename T, int NT, int U>
+// Everything row_step reads from HBM for one input row, fetched one row ahead
+// so that the loads of row t+1 are in flight while row t computes.  Upwind
+// metric candidates for both signs of the Courant number are fetched, so no
+// address depends on a loaded value.
+template <typename T, int NT>
+struct RowIn {
+  T area, cyu, cxu, mfy, mfx, rarea;
+  T ya, yb, yc, yd, ye;  // FP32: gyp gym; FP64: dya(j-1) sg3(j-1) dya(j) sg1(j) dx
+  T xa, xb, xc, xd, xe;  // FP32: gxp gxm; FP64: dxa(i-1) sg2(i-1) dxa(i) sg0(i) dy
+  double dp1;
+  T qy[NT], qx[NT];
+};
+
+template <typename T>
+struct RowIdx {
+  int r, rc, m, mc, fc;
+  long long orow, corow, co;  // row offsets are wave-uniform; co adds the lane
+  bool row_in, out_row;
+};
+
+template <typename T>
+__device__ __forceinline__ RowIdx<T> row_idx(const FusedArgs<T> &a, const StripCtx<T> &sc, int t) {
+  RowIdx<T> x;
+  x.r = sc.j0 - NG + t;
+  x.rc = x.r > a.jed ? a.jed : x.r;
+  x.m = x.r - 3;
+  x.mc = x.m < a.js ? a.js : (x.m > a.je ? a.je : x.m);
+  const int f = x.r - 2;
+  x.fc = f < a.js ? a.js : (f > a.je + 1 ? a.je + 1 : f);
+  x.orow = (long long)(x.rc - a.jsd) * a.nid;
+  x.row_in = x.rc >= a.js && x.rc <= a.je;
+  x.out_row = t >= 2 * NG && x.m >= sc.j0 && x.m < sc.j0 + a.H && x.m <= a.je;
+  x.corow = (long long)(x.mc - a.jsd) * a.nid;
+  x.co = x.corow + sc.lo_q;
+  return x;
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void row_fetch(const FusedArgs<T> &a, const StripCtx<T> &sc,
+                                          const TrState<T> (&st)[NT], const bool (&act)[NT], int t,
+                                          RowIn<T, NT> &in) {
+  constexpr bool fp64 = sizeof(T) == 8;
+  const RowIdx<T> x = row_idx(a, sc, t);
+  const int nid = a.nid;
+  // wave-uniform row bases, indexed by the lane's 32-bit column offset
+  in.area = (a.area + x.orow)[sc.lo_q];
+  in.cyu = (sc.cyk + (long long)(x.fc - a.js) * nid)[sc.lo_q];
+  const long long frow = (long long)(x.fc - a.jsd) * nid;
+  const long long gxrow = (long long)(x.rc - a.jsd) * (nid + 1) + (a.is - a.isd);
+  const long long urow = x.orow + (a.is - a.isd);
+  if constexpr (fp64) {
+    in.ya = (a.dya + frow - nid)[sc.lo_q];
+    in.yb = (a.sin_sg + 3 * a.ms2 + frow - nid)[sc.lo_q];
+    in.yc = (a.dya + frow)[sc.lo_q];
+    in.yd = (a.sin_sg + 1 * a.ms2 + frow)[sc.lo_q];
+    in.ye = (a.dx + frow)[sc.lo_q];
+    in.xa = (a.dxa + urow - 1)[sc.lo_fx];
+    in.xb = (a.sin_sg + 2 * a.ms2 + urow - 1)[sc.lo_fx];
+    in.xc = (a.dxa + urow)[sc.lo_fx];
+    in.xd = (a.sin_sg + 0 * a.ms2 + urow)[sc.lo_fx];
+    in.xe = (a.dy + gxrow)[sc.lo_fx];
+  } else {
+    in.ya = (a.gyp + frow)[sc.lo_q];
+    in.yb = (a.gym + frow)[sc.lo_q];
+    in.xa = (a.gxp + gxrow)[sc.lo_fx];
+    in.xb = (a.gxm + gxrow)[sc.lo_fx];
+  }
+  in.cxu = (sc.cxk + (long long)(x.rc - a.jsd) * sc.nix)[sc.lo_fx];
+  in.mfy = (sc.mfyk + (long long)(x.fc - a.js) * sc.niy)[sc.lo_mx];
+  in.mfx = (sc.mfxk + (long long)(x.mc - a.js) * sc.nix)[sc.lo_fx];
+  in.rarea = T(0);
+  in.dp1 = 0.0;
+  if (x.out_row) {
+    in.rarea = (a.rarea + x.corow)[sc.lo_q];
+    in.dp1 = (sc.dp1k + x.corow)[sc.lo_q];
+  }
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    if (!act[n]) continue;
+    const TrState<T> &s = st[n];
+    // lean blocks have no halo lanes or rows: the pass input is always read
+    T qy = (!sc.edge_rows || (sc.lane_in && x.row_in)) ? (s.qin + x.orow)[sc.lo_q]
+                                                       : (s.qhalo + x.orow)[sc.lo_q];
+    T qx = qy;
+    if (sc.corner_wg && !x.row_in && !sc.lane_in) {
+      int si, sj;
+      if (corner_src(sc.ic, x.rc, a.npx, a.npy, 2, a.corners, si, sj)) {
+        qy = s.qhalo[(long long)(sj - a.jsd) * nid + (si - a.isd)];
+        corner_src(sc.ic, x.rc, a.npx, a.npy, 1, a.corners, si, sj);
+        qx = s.qhalo[(long long)(sj - a.jsd) * nid + (si - a.isd)];
+      }
+    }
+    in.qy[n] = qy;
+    in.qx[n] = qx;
+  }
+}
+
+// One input row r = j0 - 3 + t of the streaming pass; U = t mod 6.  HO: the
+// advection order when fixed at compile time (0: taken from the arguments).
+template <typename T, int NT, int U, int HO>
 __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T> &sc,
                                          LvState<T> &lv, TrState<T> (&st)[NT], const bool (&act)[NT],
-                                         int t) {
+                                         int t, const RowIn<T, NT> &in) {
   constexpr bool fp64 = sizeof(T) == 8;
   constexpr int M3 = (U + 3) % 6;  // slot of row r-3
   const int nid = a.nid;
-  const int r = sc.j0 - NG + t;
-  const int rc = r > a.jed ? a.jed : r;
-  const int m = r - 3;
-  const int mc = m < a.js ? a.js : (m > a.je ? a.je : m);
-  const int f = r - 2;
-  const int fc = f < a.js ? a.js : (f > a.je + 1 ? a.je + 1 : f);
-  const long long orow = (long long)(rc - a.jsd) * nid;  // row rc of (isd:ied, jsd:jed)
-  const bool row_in = rc >= a.js && rc <= a.je;
-  const int ord_in = a.hord == 10 ? 8 : a.hord;
+  const RowIdx<T> x = row_idx(a, sc, t);
+  const int r = x.r;
+  const int hord = HO ? HO : a.hord;
+  const int ord_in = hord == 10 ? 8 : hord;
 
   // ---- level data of this row (shared by the NT tracers)
-  const T area_r = a.area[orow + sc.lo_q];
+  const T area_r = in.area;
   lv.area_r[U] = area_r;
-  const T cyu = sc.cyk[(long long)(fc - a.js) * nid + sc.lo_q];
-  const T cys = sc.scale ? cyu * sc.frac : cyu;
+  const T cyu = in.cyu;
+  // x * (1/nsplit) with 1/1 = 1 exactly: unconditional scaling rounds the same
+  const T cys = cyu * sc.frac;
   T yfx;
   {
     const bool pos = cyu > T(0);
-    const long long fo = (long long)(fc - a.jsd) * nid + sc.lo_q;
-    if constexpr (fp64) {
-      const long long uo = pos ? fo - nid : fo;
-      yfx = cyu * a.dya[uo] * a.dx[fo] * a.sin_sg[(pos ? 3 : 1) * a.ms2 + uo];
-    } else {
-      yfx = cyu * (pos ? a.gyp[fo] : a.gym[fo]);
-    }
-    if (sc.scale) yfx = yfx * sc.frac;
+    if constexpr (fp64)
+      yfx = pos ? cyu * in.ya * in.ye * in.yb : cyu * in.yc * in.ye * in.yd;
+    else
+      yfx = cyu * (pos ? in.ya : in.yb);
+    yfx = yfx * sc.frac;
   }
-  const T cxr_u = sc.cxk[(long long)(rc - a.jsd) * sc.nix + sc.lo_fx];
+  const T cxr_u = in.cxu;
   lv.cx_r[U] = cxr_u;
-  const T cxr = sc.scale ? cxr_u * sc.frac : cxr_u;
+  const T cxr = cxr_u * sc.frac;
   T xfx;
   {
     const bool pos = cxr_u > T(0);
-    if constexpr (fp64) {
-      const long long uo = orow + sc.lo_fx + (a.is - a.isd) - (pos ? 1 : 0);
-      xfx = cxr_u * a.dxa[uo] * a.dy[(long long)(rc - a.jsd) * (nid + 1) + sc.lo_fx + (a.is - a.isd)] *
-            a.sin_sg[(pos ? 2 : 0) * a.ms2 + uo];
-    } else {
-      const long long fo = (long long)(rc - a.jsd) * (nid + 1) + sc.lo_fx + (a.is - a.isd);
-      xfx = cxr_u * (pos ? a.gxp[fo] : a.gxm[fo]);
-    }
-    if (sc.scale) xfx = xfx * sc.frac;
+    if constexpr (fp64)
+      xfx = pos ? cxr_u * in.xa * in.xe * in.xb : cxr_u * in.xc * in.xe * in.xd;
+    else
+      xfx = cxr_u * (pos ? in.xa : in.xb);
+    xfx = xfx * sc.frac;
   }
   const T xfxp = from_right(xfx);
   const T rax = area_r + xfx - xfxp;
@@ -102,19 +189,17 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     rray = __builtin_amdgcn_rcpf(ray);
   }
   const T cxm_u = lv.cx_r[M3];  // cx of row m (row r-3 was loaded 3 steps ago)
-  const T cxm = sc.scale ? cxm_u * sc.frac : cxm_u;
-  const T mfy_f = sc.mfyk[(long long)(fc - a.js) * sc.niy + sc.lo_mx];
-  const T mfys = sc.scale ? mfy_f * sc.frac : mfy_f;
-  const T mfx_m = sc.mfxk[(long long)(mc - a.js) * sc.nix + sc.lo_fx];
-  const T mfxs = sc.scale ? mfx_m * sc.frac : mfx_m;
+  const T cxm = cxm_u * sc.frac;
+  const T mfys = in.mfy * sc.frac;
+  const T mfxs = in.mfx * sc.frac;
   const T mfxp = from_right(mfxs);
   T dxa_m = T(1), dxa_r = T(1);
   if (sc.edge_strip) {
-    dxa_m = a.dxa[(long long)(mc - a.jsd) * nid + sc.lo_q];
-    dxa_r = a.dxa[orow + sc.lo_q];
+    dxa_m = a.dxa[(long long)(x.mc - a.jsd) * nid + sc.lo_q];
+    dxa_r = a.dxa[x.orow + sc.lo_q];
   }
   // cube-edge role of cell r-2 along the columns (uniform across the wave)
-  const int role_y = edge_role(r - 2, a.js, a.je, a.npy, a.edge);
+  const int role_y = sc.edge_rows ? edge_role(r - 2, a.js, a.je, a.npy, a.edge) : 0;
   T dd0 = T(0), dd1 = T(0), dd2 = T(0), dd3 = T(0), dd4 = T(0);
   if (role_y != 0) {
     auto dya_at = [&](int jj) {
@@ -128,17 +213,14 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     dd4 = dya_at(r);
   }
   // output row: thickness and area factors shared by the tracers
-  const bool out_row = t >= 2 * NG && m >= sc.j0 && m < sc.j0 + a.H && m <= a.je;
   T rarea = T(0), d1 = T(0), rdp2 = T(0), dp2 = T(0);
-  long long co = 0;
-  if (out_row) {
-    co = (long long)(mc - a.jsd) * nid + sc.lo_q;
-    rarea = a.rarea[co];
+  if (x.out_row) {
+    rarea = in.rarea;
     const T ddiv = ((mfxs - mfxp) + (lv.mfy_prev - mfys)) * rarea;
-    T x = (T)sc.dp1k[co];
-    for (int s = 1; s < a.it; s++) x = x + ddiv;
-    d1 = x;
-    dp2 = x + ddiv;
+    T xd = (T)in.dp1;
+    for (int s = 1; s < a.it; s++) xd = xd + ddiv;
+    d1 = xd;
+    dp2 = xd + ddiv;
     if constexpr (!fp64) rdp2 = __builtin_amdgcn_rcpf(dp2);
   }
 
@@ -147,17 +229,7 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   for (int n = 0; n < NT; n++) {
     if (!act[n]) continue;
     TrState<T> &s = st[n];
-    const long long o = orow + sc.lo_q;
-    T qy = (sc.lane_in && row_in) ? s.qin[o] : s.qhalo[o];
-    T qx = qy;
-    if (sc.corner_wg && !row_in && !sc.lane_in) {
-      int si, sj;
-      if (corner_src(sc.ic, rc, a.npx, a.npy, 2, a.corners, si, sj)) {
-        qy = s.qhalo[(long long)(sj - a.jsd) * nid + (si - a.isd)];
-        corner_src(sc.ic, rc, a.npx, a.npy, 1, a.corners, si, sj);
-        qx = s.qhalo[(long long)(sj - a.jsd) * nid + (si - a.isd)];
-      }
-    }
+    const T qy = in.qy[n], qx = in.qx[n];
     // inner y-sweep at face r-2 and q_i of row r-3
     const T fy2 = ypipe_step<T, U>(s.py, qy, cys, ord_in, role_y, dd0, dd1, dd2, dd3, dd4);
     const T qm = s.py.w[M3];
@@ -167,7 +239,7 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     else
       q_i = (qm * area_m + lv.yfx_prev * s.fy2_prev - yfx * fy2) * rray;
     // outer x-sweep of row r-3, inner x-sweep of row r, q_j of row r
-    const T fx = xsweep(q_i, cxm, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, a.hord, dxa_m);
+    const T fx = xsweep(q_i, cxm, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, hord, dxa_m);
     const T fx2 = xsweep(qx, cxr, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, ord_in, dxa_r);
     s.fx2_r[U] = fx2;
     const T fx2p = from_right(fx2);
@@ -177,15 +249,16 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     else
       q_j = (qx * area_r + xfx * fx2 - xfxp * fx2p) * rrax;
     // outer y-sweep at face r-2, fluxes, update of row r-3
-    const T fy = ypipe_step<T, U>(s.pj, q_j, cys, a.hord, role_y, dd0, dd1, dd2, dd3, dd4);
+    const T fy = ypipe_step<T, U>(s.pj, q_j, cys, hord, role_y, dd0, dd1, dd2, dd3, dd4);
     const T Fy = T(0.5) * (fy + fy2) * mfys;
     const T Fx = T(0.5) * (fx + s.fx2_r[M3]) * mfxs;
     const T Fxp = from_right(Fx);
-    if (out_row && sc.col_out) {
+    if (x.out_row && sc.col_out) {
+      T *orow = s.qout + x.corow;
       if constexpr (fp64)
-        s.qout[co] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) / dp2;
+        orow[sc.lo_q] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) / dp2;
       else
-        s.qout[co] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) * rdp2;
+        orow[sc.lo_q] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) * rdp2;
     }
     s.fy2_prev = fy2;
     s.Fy_prev = Fy;
@@ -194,15 +267,26 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   lv.mfy_prev = mfys;
 }
 
-template <typename T, int W, int NT>
-__global__ void __launch_bounds__(FT_LANES *W) k_fused_tp(FusedArgs<T> a) {
+// EDGE = false: the lean variant for the interior blocks (no cube-edge,
+// corner or halo-lane code is compiled); EDGE = true: every other block.
+template <typename T, int W, int NT, int HO, bool EDGE>
+__global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1) ? 1 : 3) k_fused_tp(FusedArgs<T> a) {
   const int lane = threadIdx.x & (FT_LANES - 1), wave = threadIdx.x / FT_LANES;
   const int k = blockIdx.y;
   const int nspl = a.nsplit[k];
   // whole block: this level is done, or needs more passes than are launched
   // (reported by the caller's nsplit check; nothing is written for it)
   if (a.it > nspl || nspl > a.cap) return;
-  const int strip = blockIdx.x % a.nstrip, vb = blockIdx.x / a.nstrip;
+  int strip, vb;
+  if constexpr (EDGE) {
+    strip = blockIdx.x % a.nstrip;
+    vb = blockIdx.x / a.nstrip;
+    if (strip >= a.s_lo && strip <= a.s_hi && vb >= a.v_lo && vb <= a.v_hi) return;  // lean variant's
+  } else {
+    const int ns = a.s_hi - a.s_lo + 1;
+    strip = a.s_lo + (int)(blockIdx.x % ns);
+    vb = a.v_lo + (int)(blockIdx.x / ns);
+  }
   const int i0 = a.is + strip * FT_BX;
   StripCtx<T> sc;
   sc.j0 = a.js + vb * a.H;
@@ -214,8 +298,9 @@ __global__ void __launch_bounds__(FT_LANES *W) k_fused_tp(FusedArgs<T> a) {
   sc.lo_mx = (sc.i > a.ie ? a.ie : (sc.i < a.is ? a.is : sc.i)) - a.is;
   sc.col_out = lane >= NG && lane < NG + FT_BX && sc.i <= a.ie;
   sc.lane_in = sc.i >= a.is && sc.i <= a.ie;
-  sc.edge_strip = a.edge && (i0 - NG <= 2 || i0 - NG + FT_LANES - 1 >= a.npx - 2);
-  sc.corner_wg = a.corners != 0u && (strip == 0 || strip == a.nstrip - 1) && (vb == 0 || vb == a.nvblk - 1);
+  sc.edge_strip = EDGE && a.edge && (i0 - NG <= 2 || i0 - NG + FT_LANES - 1 >= a.npx - 2);
+  sc.edge_rows = EDGE;
+  sc.corner_wg = EDGE && a.corners != 0u && (strip == 0 || strip == a.nstrip - 1) && (vb == 0 || vb == a.nvblk - 1);
   sc.scale = nspl > 1;
   sc.frac = T(1) / T(nspl);
   sc.nix = a.ie - a.is + 2;
@@ -247,10 +332,19 @@ __global__ void __launch_bounds__(FT_LANES *W) k_fused_tp(FusedArgs<T> a) {
     LvState<T> lv;
     lv.yfx_prev = lv.mfy_prev = T(0);
     const bool any = act[0];
+    // rows are fetched one ahead; the bare s_barrier keeps the waves of the
+    // workgroup (same level data, L1 reuse) in step without waiting for the
+    // loads in flight -- the waves share no memory
+    // double buffer by row parity (6 rows per unrolled step: parity is U's)
+    RowIn<T, NT> buf[2];
+    if (any) row_fetch<T, NT>(a, sc, st, act, 0, buf[0]);
     for (int t0 = 0; t0 < nrow; t0 += 6) {
-#define FV3_ROW(U)                                                          \
-  if (any && t0 + U < nrow) row_step<T, NT, U>(a, sc, lv, st, act, t0 + U); \
-  if (W > 1) __syncthreads();
+#define FV3_ROW(U)                                                                   \
+  if (any && t0 + U < nrow) {                                                        \
+    if (t0 + U + 1 < nrow) row_fetch<T, NT>(a, sc, st, act, t0 + U + 1, buf[(U + 1) & 1]); \
+    row_step<T, NT, U, HO>(a, sc, lv, st, act, t0 + U, buf[U & 1]);                  \
+  }                                                                                  \
+  if (W > 1) __builtin_amdgcn_s_barrier();
       FV3_ROW(0) FV3_ROW(1) FV3_ROW(2) FV3_ROW(3) FV3_ROW(4) FV3_ROW(5)
 #undef FV3_ROW
     }

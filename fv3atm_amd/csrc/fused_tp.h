@@ -42,6 +42,9 @@ struct FusedArgs {
   unsigned corners;
   bool edge;
   int npz, nq, hord, it, cap, H, nstrip, nvblk;
+  // blocks (strip, vblock) inside [s_lo, s_hi] x [v_lo, v_hi] touch no
+  // cube-edge cell, corner or halo lane: the lean kernel variant takes them
+  int s_lo, s_hi, v_lo, v_hi;
   const int *nsplit;  // per level
   // fields (device)
   const T *q_in;  // step input (also the frozen halo source)
@@ -55,8 +58,10 @@ struct FusedArgs {
 };
 
 // ---- cross-lane shifts: value of lane l-1 / l+1 (DPP wave_shr:1 / wave_shl:1)
-__device__ __forceinline__ int dpp_from_left(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_from_right(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+// (bound_ctrl: the lane past the wave edge reads 0, so no destination
+// initialisation is needed; that lane is a halo lane whose result is unused)
+__device__ __forceinline__ int dpp_from_left(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_from_right(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true); }
 
 template <typename T>
 __device__ __forceinline__ T from_left(T v) {
@@ -186,11 +191,20 @@ __device__ __forceinline__ void cell_coef(T qm2, T qm1, T q0, T qp1, T qp2, T dm
   }
 }
 
-// upwind flux at a face from the upwind cell's (q, bl, br)
+// Upwind flux at a face with Courant number c, from the cells on its left
+// (qL, blL, brL) and right (qR, blR, brR).  Upstream writes the two branches
+//   c > 0:  qL + (1 - c) (brL - c (blL + brL))
+//   else:   qR + (1 + c) (blR + c (blR + brR));
+// with a = |c| both are  qu + (1 - a) (x - a (blu + bru))  where u is the
+// upwind cell and x its downwind-side perturbation: 1 - a equals 1 + c and
+// a S equals -(c S) exactly, so this branch-free form rounds the same.
 template <typename T>
-__device__ __forceinline__ T face_flux(T c, T qu, T blu, T bru) {
-  if (c > T(0)) return qu + (T(1) - c) * (bru - c * (blu + bru));
-  return qu + (T(1) + c) * (blu + c * (blu + bru));
+__device__ __forceinline__ T face_flux2(T c, T qL, T blL, T brL, T qR, T blR, T brR) {
+  const bool pos = c > T(0);
+  const T qu = pos ? qL : qR, blu = pos ? blL : blR, bru = pos ? brL : brR;
+  const T x = pos ? bru : blu;
+  const T ac = fabs(c);
+  return qu + (T(1) - ac) * (x - ac * (blu + bru));
 }
 
 // x-direction PPM across the lanes of a row: flux at the face left of each
@@ -217,7 +231,7 @@ __device__ __forceinline__ T xsweep(T q, T c, int i, int is, int ie, int npx, bo
     if (role != 0) edge_coef(qm2, qm1, q, qp1, qp2, dmm1, dm, dmp1, al, alp1, role, d0, d1, dxl, d3, d4, bl, br);
   }
   const T blm = from_left(bl), brm = from_left(br);
-  return c > T(0) ? face_flux(c, qm1, blm, brm) : face_flux(c, q, bl, br);
+  return face_flux2(c, qm1, blm, brm, q, bl, br);
 }
 
 // Sliding y-direction PPM over a column: fed one cell per row, yields the
@@ -246,7 +260,7 @@ __device__ __forceinline__ T ypipe_step(YPipe<T> &p, T qnew, T c, int iord, int 
   if (role != 0)
     edge_coef(p.w[s4], p.w[s3], p.w[s2], p.w[s1], p.w[s0], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2],
               p.al[s1], role, d0, d1, d2, d3, d4, p.bl[s2], p.br[s2]);
-  return c > T(0) ? face_flux(c, p.w[s3], p.bl[s3], p.br[s3]) : face_flux(c, p.w[s2], p.bl[s2], p.br[s2]);
+  return face_flux2(c, p.w[s3], p.bl[s3], p.br[s3], p.w[s2], p.bl[s2], p.br[s2]);
 }
 
 }  // namespace fv3
