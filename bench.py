@@ -86,6 +86,42 @@ def run_form(fa, ctx, form, d, case, steps, warmup, cap, torch, timing=False):
     return ms, t_adv, t_rem
 
 
+def run_graph(fa, ctx, d, case, steps, warmup, cap, torch):
+    """Capture one step per buffer direction in a hipGraph and time replays."""
+    fa.set_graph_safe(ctx, True)
+
+    def one(q_in, q_out, stream):
+        fa.step_dev(ctx, fa.FORM_FUSED32, case.nq, q_in, q_out, d["pt"], d["dp1"], d["delp_out"],
+                    d["cx"], d["cy"], d["mfx"], d["mfy"], d["ak"], d["bk"], case.ptop, case.hord,
+                    case.kord_tr, case.kord_tm, 184.0, cap, stream)
+
+    a, b = d["q_in"], d["q_out"]
+    one(a, b, None)  # graph-safe warm-up outside capture
+    torch.cuda.synchronize()
+    graphs = []
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for src, dst in ((a, b), (b, a)):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=s):
+                one(src, dst, torch.cuda.current_stream().cuda_stream)
+            graphs.append(g)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    for i in range(warmup):
+        graphs[i % 2].replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        graphs[(warmup + i) % 2].replay()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    bad = fa.step_status(ctx)
+    fa.set_graph_safe(ctx, False)
+    return ms, bad
+
+
 def cpu_baseline(case):
     """FP64 oracle on the full tile, every level, one tracer (+ pt remap)."""
     from oracle import oracle
@@ -119,6 +155,9 @@ def main():
                     help="first run a 1 GiB copy kernel (reference for HBM byte counters)")
     ap.add_argument("--launch", type=int, default=-1,
                     help="fused launch configuration; -1 tunes over all of them first")
+    ap.add_argument("--rows", type=int, default=0, help="rows per block (0: default or tuned)")
+    ap.add_argument("--graph", action="store_true",
+                    help="time the step as hipGraph replays (graph-safe mode)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -204,22 +243,39 @@ def main():
         torch.cuda.empty_cache()
         d = device_inputs(case, torch, torch.float32)
     fa.set_timing(ctx, True)
-    launch = args.launch
-    if launch < 0:  # untimed tuning of the launch shape on this workload
+    launch, rows = args.launch, args.rows
+    if launch < 0:  # untimed tuning of the launch shape, then rows per block
         best = None
         for cfg in range(fa.N_LAUNCH_CFG):
-            fa.set_launch(ctx, cfg, 0)
+            fa.set_launch(ctx, cfg, rows)
             _, ta_c, _ = run_form(fa, ctx, fa.FORM_FUSED32, d, case, 2, 1, cap, torch, timing=True)
             t = float(np.min(ta_c))
-            log(f"launch cfg {cfg}: transport {t:.2f} ms")
+            log(f"launch cfg {cfg} rows {rows or 'default'}: transport {t:.2f} ms")
             if best is None or t < best[0]:
-                best = (t, cfg)
-        launch = best[1]
-    fa.set_launch(ctx, launch, 0)
+                best = (t, cfg, rows)
+        for r in (60, 128, 192):
+            if r == rows or r > N:
+                continue
+            fa.set_launch(ctx, best[1], r)
+            _, ta_c, _ = run_form(fa, ctx, fa.FORM_FUSED32, d, case, 2, 1, cap, torch, timing=True)
+            t = float(np.min(ta_c))
+            log(f"launch cfg {best[1]} rows {r}: transport {t:.2f} ms")
+            if t < best[0]:
+                best = (t, best[1], r)
+        launch, rows = best[1], best[2]
+    fa.set_launch(ctx, launch, rows)
     out["launch_cfg"] = launch
+    out["rows_per_block"] = rows or 90
     ms, t_adv, t_rem = run_form(fa, ctx, fa.FORM_FUSED32, d, case, args.steps, args.warmup, cap,
                                 torch, timing=True)
     log(f"fused FP32: {ms:.2f} ms/step (transport {np.mean(t_adv):.2f} ms, remap {np.mean(t_rem):.2f} ms)")
+    if args.graph:
+        ms_eager = ms
+        ms, bad = run_graph(fa, ctx, d, case, args.steps, args.warmup, cap, torch)
+        assert bad == 0, "a level needed more sub-cycles than the captured nsplit cap"
+        log(f"fused FP32 as hipGraph replays: {ms:.2f} ms/step (eager {ms_eager:.2f})")
+        out["eager_ms_per_step"] = round(ms_eager, 3)
+        out["graph"] = True
 
     cells = N * N * L * Q
     value = cells / (ms * 1e-3) / 1e9

@@ -279,7 +279,7 @@ int fv3_ctx_set_grid(fv3_ctx *h, int tile, const fv3_grid *g) {
     for (long long k = 0; k < x.n; k++) tmp[k] = (float)x.src[k];
     FV3_HIP(hipMemcpy(x.d32, tmp.data(), sizeof(float) * x.n, hipMemcpyHostToDevice));
   }
-  if (tile == 1) {  // FP32 combined factors for the fused FP32 form (single tile)
+  {  // FP32 combined upwind factors of this tile for the fused FP32 form
     const int nid = c.d.nid, njd = c.d.njd;
     const long long nx = (long long)(nid + 1) * njd, ny = (long long)nid * (njd + 1);
     std::vector<float> f(2 * nx + 2 * ny, 0.0f);
@@ -297,8 +297,9 @@ int fv3_ctx_set_grid(fv3_ctx *h, int tile, const fv3_grid *g) {
         f[2 * nx + fo] = (float)(g->dya[(long long)(j - 1) * nid + i] * dxv * g->sin_sg[3 * s2 + (long long)(j - 1) * nid + i]);
         f[2 * nx + ny + fo] = (float)(g->dya[(long long)j * nid + i] * dxv * g->sin_sg[1 * s2 + (long long)j * nid + i]);
       }
-    if (!c.gfac32) FV3_HIP(hipMalloc(&c.gfac32, sizeof(float) * f.size()));
-    FV3_HIP(hipMemcpy(c.gfac32, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+    if (!c.gfac32) FV3_HIP(hipMalloc(&c.gfac32, sizeof(float) * f.size() * c.ntiles));
+    FV3_HIP(hipMemcpy(c.gfac32 + (long long)t * f.size(), f.data(), sizeof(float) * f.size(),
+                      hipMemcpyHostToDevice));
   }
   if (tile == 1 || g->da_min < c.da_min || c.da_min == 0.0) c.da_min = g->da_min;
   unsigned corners = (g->sw_corner ? 1u : 0u) | (g->se_corner ? 2u : 0u) | (g->ne_corner ? 4u : 0u) |

@@ -15,9 +15,9 @@ static void go64(const FusedArgs<double> &a, int nblk, hipStream_t s) {
   const dim3 b(FT_LANES * 4);
   const int nin = a.s_hi >= a.s_lo && a.v_hi >= a.v_lo ? (a.s_hi - a.s_lo + 1) * (a.v_hi - a.v_lo + 1) : 0;
   if (nin > 0)
-    hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, false>), dim3((unsigned)nin, (unsigned)a.npz, 1), b, 0, s, a);
+    hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, false>), dim3((unsigned)nin, (unsigned)a.npz, (unsigned)a.ntiles), b, 0, s, a);
   if (nin < nblk)
-    hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, true>), dim3((unsigned)nblk, (unsigned)a.npz, 1), b, 0, s, a);
+    hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, true>), dim3((unsigned)nblk, (unsigned)a.npz, (unsigned)a.ntiles), b, 0, s, a);
 }
 
 int launch_fused_f64(const FusedArgs<double> &a, int nblk, hipStream_t s) {
@@ -45,7 +45,6 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   const Dims &d = c.d;
   const int npz = c.npz;
   hipStream_t s = c.stream;
-  if (c.ntiles != 1) return set_error("the fused forms take a single-tile context");
   if (hord < 7 || hord > 13) return set_error("hord=%d: only the monotone orders 7..13 are implemented", hord);
   if (c.graph_safe && nsplit_cap <= 0)
     return set_error("graph-safe steps need nsplit_cap > 0 (no device-to-host read)");
@@ -72,7 +71,8 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     cap = *std::max_element(h.begin(), h.end());
     if (cap < 1) return set_error("nsplit < 1 from a non-finite Courant number");
   }
-  const long long qsize = d.s2 * npz * nq;
+  const int nt = c.ntiles;
+  const long long qsize = d.s2 * npz * nq * nt;  // all tiles
   const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
   // The FP32 form remaps out of place: the last transport pass of every
   // level lands in q_tmp (the caller's q_out serves as the alternate pass
@@ -82,7 +82,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   RemapCols cols{};
   if (fast_remap) {
     const size_t b_q = ((sizeof(T) * qsize) + 255) & ~(size_t)255;
-    const size_t b_pt = pt ? ((sizeof(T) * d.s2 * npz) + 255) & ~(size_t)255 : 0;
+    const size_t b_pt = pt ? ((sizeof(T) * d.s2 * npz * nt) + 255) & ~(size_t)255 : 0;
     char *sc = (char *)ctx_scratch(c, b_q + b_pt + remap_cols_bytes(npz, (long long)ni * nj));
     if (!sc) return 1;
     q_tmp = (T *)sc;
@@ -100,6 +100,13 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   a.corners = d.bounded ? 0u : d.corners;
   a.edge = d.edge;
   a.npz = npz; a.nq = nq; a.hord = hord; a.cap = cap;
+  a.ntiles = nt;
+  a.tq = d.s2 * npz * nq;
+  a.tcx = d.sx * npz;
+  a.tcy = d.sy * npz;
+  a.tmx = d.smx * npz;
+  a.tmy = d.smy * npz;
+  a.tdp = d.s2 * npz;
   a.H = std::min(nj, c.fused_h > 0 ? c.fused_h : 90);
   a.nstrip = (ni + FT_BX - 1) / FT_BX;
   a.nvblk = (nj + a.H - 1) / a.H;
@@ -134,11 +141,15 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   const GridDev<T> &g = grid_of<T>(c);
   a.area = g.area; a.rarea = g.rarea; a.dxa = g.dxa; a.dya = g.dya; a.dx = g.dx; a.dy = g.dy;
   a.sin_sg = g.sin_sg; a.ms2 = g.s2;
+  a.tgs2 = g.s2;
+  a.tgdx = g.sdx;
+  a.tgdy = g.sdy;
   if (sizeof(T) == 4) {
     a.gxp = (const T *)c.gfac32;
     a.gxm = a.gxp + (long long)(d.nid + 1) * d.njd;
     a.gyp = a.gxm + (long long)(d.nid + 1) * d.njd;
     a.gym = a.gyp + (long long)d.nid * (d.njd + 1);
+    a.tgf = 2 * (long long)(d.nid + 1) * d.njd + 2 * (long long)d.nid * (d.njd + 1);
   }
   const int nblk = a.nstrip * a.nvblk;
   const bool timing = c.timing && !c.graph_safe;
@@ -158,15 +169,19 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   }
   if (timing) FV3_HIP(hipEventRecord(c.ev[1], s));
   // remap: source thickness is dp1, target lands in delp_out
-  FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
+  FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, s));
   if (fast_remap && pt)
-    FV3_HIP(hipMemcpyAsync(pt_src, pt, sizeof(T) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
+    FV3_HIP(hipMemcpyAsync(pt_src, pt, sizeof(T) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, s));
   if (timing) FV3_HIP(hipEventRecord(c.ev[2], s));
   if constexpr (std::is_same<T, float>::value) {
     if (fast_remap) {
-      if (remap_fast_f32(c, npz, nq, ak, bk, ptop, dp1, delp_out, q_tmp, q_out, pt_src, pt, kord_tr,
-                         kord_tm, t_min, cols))
-        return 1;
+      for (int t = 0; t < nt; t++) {  // column data of one tile at a time
+        const long long tq = a.tq, tl = d.s2 * npz;
+        if (remap_fast_f32(c, npz, nq, ak, bk, ptop, dp1 + t * tl, delp_out + t * tl, q_tmp + t * tq,
+                           q_out + t * tq, pt_src ? pt_src + t * tl : nullptr, pt ? pt + t * tl : nullptr,
+                           kord_tr, kord_tm, t_min, cols))
+          return 1;
+      }
     }
   }
   if (!fast_remap) {

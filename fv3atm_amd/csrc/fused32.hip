@@ -10,9 +10,9 @@ static void go2(const FusedArgs<float> &a, int nblk, hipStream_t s) {
   const dim3 b(FT_LANES * W);
   const int nin = a.s_hi >= a.s_lo && a.v_hi >= a.v_lo ? (a.s_hi - a.s_lo + 1) * (a.v_hi - a.v_lo + 1) : 0;
   if (nin > 0)
-    hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, false>), dim3((unsigned)nin, (unsigned)a.npz, 1), b, 0, s, a);
+    hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, false>), dim3((unsigned)nin, (unsigned)a.npz, (unsigned)a.ntiles), b, 0, s, a);
   if (nin < nblk)
-    hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, true>), dim3((unsigned)nblk, (unsigned)a.npz, 1), b, 0, s, a);
+    hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, true>), dim3((unsigned)nblk, (unsigned)a.npz, (unsigned)a.ntiles), b, 0, s, a);
 }
 
 template <int W, int NT>

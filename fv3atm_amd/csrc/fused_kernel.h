@@ -31,6 +31,7 @@ struct TrState {
   T fx2_r[6];
   T fy2_prev, Fy_prev;
   const T *qin, *qhalo;
+  const T *qsrc0;  // pass input of tile 1 at this tracer and level (six-tile halos)
   T *qout;
 };
 
@@ -125,7 +126,27 @@ __device__ __forceinline__ void row_fetch(const FusedArgs<T> &a, const StripCtx<
     T qy = (!sc.edge_rows || (sc.lane_in && x.row_in)) ? (s.qin + x.orow)[sc.lo_q]
                                                        : (s.qhalo + x.orow)[sc.lo_q];
     T qx = qy;
-    if (sc.corner_wg && !x.row_in && !sc.lane_in) {
+    if (sc.edge_rows && a.ntiles > 1 && !(sc.lane_in && x.row_in)) {
+      // six tiles: edge-halo cells come from the neighbour face; corner
+      // cells take their copy_corners source first (dir 2 for the y-sweep,
+      // dir 1 for the x-sweep), which is an edge-halo cell
+      const int n = a.ie;
+      const int tile = blockIdx.z + 1;
+      auto from_cube = [&](int ci, int cj) -> T {
+        int ts, si, sj;
+        cube_halo_src(tile, ci, cj, n, ts, si, sj);
+        return (s.qsrc0 + (long long)(ts - 1) * a.tq)[(long long)(sj - a.jsd) * nid + (si - a.isd)];
+      };
+      int si, sj;
+      if (corner_src(sc.ic, x.rc, a.npx, a.npy, 2, a.corners, si, sj)) {
+        qy = from_cube(si, sj);
+        corner_src(sc.ic, x.rc, a.npx, a.npy, 1, a.corners, si, sj);
+        qx = from_cube(si, sj);
+      } else {
+        qy = from_cube(sc.ic, x.rc);
+        qx = qy;
+      }
+    } else if (sc.corner_wg && !x.row_in && !sc.lane_in) {
       int si, sj;
       if (corner_src(sc.ic, x.rc, a.npx, a.npy, 2, a.corners, si, sj)) {
         qy = s.qhalo[(long long)(sj - a.jsd) * nid + (si - a.isd)];
@@ -277,6 +298,27 @@ __global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1) ? 1 : 3) k_fused
   // whole block: this level is done, or needs more passes than are launched
   // (reported by the caller's nsplit check; nothing is written for it)
   if (a.it > nspl || nspl > a.cap) return;
+  if (a.ntiles > 1) {  // this block's tile: offset every per-tile array
+    const long long t = blockIdx.z;
+    a.cx += t * a.tcx;
+    a.cy += t * a.tcy;
+    a.mfx += t * a.tmx;
+    a.mfy += t * a.tmy;
+    a.dp1 += t * a.tdp;
+    a.area += t * a.tgs2;
+    a.rarea += t * a.tgs2;
+    a.dxa += t * a.tgs2;
+    a.dya += t * a.tgs2;
+    a.sin_sg += t * 9 * a.tgs2;
+    a.dx += t * a.tgdx;
+    a.dy += t * a.tgdy;
+    if (sizeof(T) == 4) {
+      a.gxp += t * a.tgf;
+      a.gxm += t * a.tgf;
+      a.gyp += t * a.tgf;
+      a.gym += t * a.tgf;
+    }
+  }
   int strip, vb;
   if constexpr (EDGE) {
     strip = blockIdx.x % a.nstrip;
@@ -324,9 +366,11 @@ __global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1) ? 1 : 3) k_fused
       const int iq = rd * per_round + wave * NT + n;
       act[n] = iq < a.nq;
       const long long qoff = (long long)(act[n] ? iq : 0) * a.npz * a.s2 + (long long)k * a.s2;
-      st[n].qin = in + qoff;
-      st[n].qhalo = a.q_in + qoff;
-      st[n].qout = out + qoff;
+      const long long toff = (long long)blockIdx.z * a.tq;
+      st[n].qsrc0 = in + qoff;
+      st[n].qin = in + toff + qoff;
+      st[n].qhalo = a.q_in + toff + qoff;
+      st[n].qout = out + toff + qoff;
       st[n].fy2_prev = st[n].Fy_prev = T(0);
     }
     LvState<T> lv;

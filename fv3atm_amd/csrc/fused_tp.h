@@ -45,6 +45,11 @@ struct FusedArgs {
   // blocks (strip, vblock) inside [s_lo, s_hi] x [v_lo, v_hi] touch no
   // cube-edge cell, corner or halo lane: the lean kernel variant takes them
   int s_lo, s_hi, v_lo, v_hi;
+  // six tiles (blockIdx.z = tile): per-tile strides of q, winds, dp1, the
+  // metrics and the FP32 factors; halo cells are read from the neighbour
+  // tile's interior in the pass input buffer
+  int ntiles;
+  long long tq, tcx, tcy, tmx, tmy, tdp, tgs2, tgdx, tgdy, tgf;
   const int *nsplit;  // per level
   // fields (device)
   const T *q_in;  // step input (also the frozen halo source)
@@ -118,6 +123,32 @@ __device__ __forceinline__ T edge_face(T a, T b, T c, T e, T da, T db, T dc, T d
   xt = fmx(xt, fmn(fmn3(a, b, c), e));
   xt = fmn(xt, fmx(fmx3(a, b, c), e));
   return xt;
+}
+
+// Cube-sphere edge halo: the source (tile ts, cell si, sj; 1-based tiles) of
+// edge-halo cell (i, j) of tile t on a cube of n x n tiles, for the FMS
+// mosaic's face orientation (odd and even faces alternate orientation).
+// Exactly one of i, j lies outside 1..n.
+__device__ __host__ __forceinline__ void cube_halo_src(int t, int i, int j, int n, int &ts, int &si, int &sj) {
+  auto nxt = [](int tt, int s) { return ((tt - 1 + s) % 6 + 6) % 6 + 1; };
+  const bool odd = (t & 1) != 0;
+  if (i > n) {  // east, depth e, along a
+    const int e = i - n, a = j;
+    if (odd) { ts = nxt(t, 1); si = e; sj = a; }
+    else { ts = nxt(t, 2); si = n + 1 - a; sj = e; }
+  } else if (j > n) {  // north
+    const int e = j - n, a = i;
+    if (odd) { ts = nxt(t, 2); si = e; sj = n + 1 - a; }
+    else { ts = nxt(t, 1); si = a; sj = e; }
+  } else if (i < 1) {  // west
+    const int e = 1 - i, a = j;
+    if (odd) { ts = nxt(t, -2); si = n + 1 - a; sj = n + 1 - e; }
+    else { ts = nxt(t, -1); si = n + 1 - e; sj = a; }
+  } else {  // south
+    const int e = 1 - j, a = i;
+    if (odd) { ts = nxt(t, -1); si = a; sj = n + 1 - e; }
+    else { ts = nxt(t, -2); si = n + 1 - e; sj = n + 1 - a; }
+  }
 }
 
 // Cube-edge role of cell m along a line: 0 none, 1..3 cells 0,1,2, 4..6 cells

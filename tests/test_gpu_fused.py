@@ -62,6 +62,9 @@ CASES = {
     "C100L6Q7jet": lambda: cases.make_case("C100L6Q7", n=100, npz=6, nq=7, jet=True,
                                            courant=1.9, delp_kind=1, delp_amp=(0.05, 0.15),
                                            seed=11, with_pt=True),
+    # the cubed sphere: edge halos and corners read from the neighbour faces
+    "6xC32L8Q2": lambda: cases.make_case("6xC32L8Q2", n=32, npz=8, nq=2, ntiles=6, courant=1.7,
+                                         delp_kind=1, delp_amp=(0.05, 0.15), seed=17, with_pt=True),
 }
 
 
@@ -93,7 +96,7 @@ def test_step_fused32_error(gpu_ready, name):
     got = run_step(case, fa.FORM_FUSED32)
     errs = []
     for iq in range(case.nq):
-        e = rel_l2(interior(got["q"][..., iq]), interior(ref["q"][..., iq]))
+        e = rel_l2(interior(got["q"][:, :, :, iq]), interior(ref["q"][:, :, :, iq]))
         errs.append(e)
     print(name, "FP32 rel L2 per tracer", ["%.2e" % e for e in errs], flush=True)
     assert max(errs) <= 1e-5
@@ -112,13 +115,13 @@ def test_step_nsplit_cap(gpu_ready):
 
 
 def test_step_interior_blocks(gpu_ready):
-    # C192 with 30-row blocks: strips 1-2 x blocks 1-5 of every level run the
+    # C192 L4 with 30-row blocks: strips 1-2 x blocks 1-5 of every level run the
     # lean interior kernel, the frame runs the cube-edge kernel; both must
     # reproduce the oracle bit for bit (FP64) and to 1e-5 (FP32)
-    case = cases.make_case("C192L3Q2jet", n=192, npz=3, nq=2, jet=True, courant=2.1, delp_kind=1,
+    case = cases.make_case("C192L4Q2jet", n=192, npz=4, nq=2, jet=True, courant=2.1, delp_kind=1,
                            delp_amp=(0.05, 0.2), seed=13, with_pt=True)
     ref = reference(case)
-    print("C192L3Q2jet nsplit", ref["nsplit"].tolist(), flush=True)
+    print("C192L4Q2jet nsplit", ref["nsplit"].tolist(), flush=True)
     got = run_step(case, fa.FORM_FUSED64, rows=30)
     assert_close(interior(got["q"]), interior(ref["q"]), 1e-12, "interior-block q", floor=1e-300)
     got32 = run_step(case, fa.FORM_FUSED32, rows=30)
@@ -127,3 +130,19 @@ def test_step_interior_blocks(gpu_ready):
         print(f"FP32 tracer {iq}: rel L2 {e:.2e}")
         assert e <= 1e-5
 
+
+
+def test_step_six_tiles_interior_blocks(gpu_ready):
+    # six C128 faces with 30-row blocks: lean interior blocks on every face,
+    # cube-edge frames reading their halos from the neighbour faces
+    case = cases.make_case("6xC128L4Q2", n=128, npz=4, nq=2, ntiles=6, courant=2.2, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=19, with_pt=True)
+    ref = reference(case)
+    print("6xC128L4Q2 nsplit", ref["nsplit"].tolist(), flush=True)
+    got = run_step(case, fa.FORM_FUSED64, rows=30)
+    assert_close(interior(got["q"]), interior(ref["q"]), 1e-12, "six-tile q", floor=1e-300)
+    got32 = run_step(case, fa.FORM_FUSED32, rows=30)
+    for iq in range(case.nq):
+        e = rel_l2(interior(got32["q"][..., iq, :]), interior(ref["q"][..., iq, :]))
+        print(f"FP32 tracer {iq}: rel L2 {e:.2e}")
+        assert e <= 1e-5
