@@ -122,6 +122,26 @@ def run_graph(fa, ctx, d, case, steps, warmup, cap, torch):
     return ms, bad
 
 
+def profiled_traffic(n, npz, nq, cfg, rows):
+    """HBM bytes per transport launch measured by rocprofv3 counters for this
+    workload and launch shape: the 'bench-traffic:' line of a summary under
+    profiles/ (FETCH_SIZE + WRITE_SIZE of every k_fused_tp dispatch of one
+    step, each scaled by the same counter's reading for a copy kernel of known
+    size, divided by the step's sub-cycle passes).  None when no summary
+    matches."""
+    import glob
+    want = dict(n=str(n), npz=str(npz), nq=str(nq), cfg=str(cfg), rows=str(rows))
+    for p in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(p) as f:
+            for line in f:
+                if not line.startswith("bench-traffic:"):
+                    continue
+                kv = dict(t.split("=", 1) for t in line.split()[1:] if "=" in t)
+                if all(kv.get(k) == v for k, v in want.items()):
+                    return int(float(kv["bytes_per_pass"])), os.path.relpath(p, ROOT)
+    return None, None
+
+
 def cpu_baseline(case):
     """FP64 oracle on the full tile, every level, one tracer (+ pt remap)."""
     from oracle import oracle
@@ -283,8 +303,15 @@ def main():
     b_adv = S * N * N * (2 * w * Q + 24)
     b_rem = N * N * ((Q + 1) * 2 * w * L + 16 * (L + 1))
     ta = float(np.mean(t_adv)) * 1e-3
+    # one launch = one sub-cycle pass over every level (the interior and frame
+    # grids of k_fused_tp back to back); cap passes per step
+    traffic, tsrc = profiled_traffic(N, L, Q, launch, rows or 90)
     roof = dict(bound="hbm", achieved=round(b_adv / ta / 1e9, 1), peak=HBM_PEAK / 1e9,
-                unit="GB/s", frac=round(b_adv / ta / HBM_PEAK, 4), traffic=None)
+                unit="GB/s", frac=round(b_adv / ta / HBM_PEAK, 4), traffic=traffic)
+    out["algorithmic_bytes_per_launch"] = b_adv // cap
+    out["launches_per_step"] = cap
+    if tsrc:
+        out["traffic_source"] = tsrc
     out_line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-tracer updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),

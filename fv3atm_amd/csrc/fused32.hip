@@ -27,12 +27,14 @@ static int go(const FusedArgs<float> &a, int nblk, hipStream_t s) {
 
 int launch_fused_f32(const FusedArgs<float> &a, int nblk, int cfg, hipStream_t s) {
   switch (cfg) {
-    case 0: return go<4, 2>(a, nblk, s);
+    case 0: return go<4, 1>(a, nblk, s);
     case 1: return go<8, 1>(a, nblk, s);
-    case 2: return go<4, 1>(a, nblk, s);
+    case 2: return go<4, 2>(a, nblk, s);
     case 3: return go<2, 2>(a, nblk, s);
     case 4: return go<6, 1>(a, nblk, s);
     case 5: return go<1, 2>(a, nblk, s);
+    case 6: return go<1, 1>(a, nblk, s);
+    case 7: return go<2, 1>(a, nblk, s);
     default: return set_error("unknown fused FP32 launch configuration %d", cfg);
   }
 }

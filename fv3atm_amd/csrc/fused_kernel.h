@@ -22,6 +22,7 @@ struct StripCtx {
   T frac;
   const T *cxk, *cyk, *mfxk, *mfyk;
   const double *dp1k;
+  long long tg, tdx, tdy, tgf;  // this tile's offsets in the metric arrays (s2, dx, dy, gfac)
 };
 
 // per tracer streaming state
@@ -87,27 +88,27 @@ __device__ __forceinline__ void row_fetch(const FusedArgs<T> &a, const StripCtx<
   const RowIdx<T> x = row_idx(a, sc, t);
   const int nid = a.nid;
   // wave-uniform row bases, indexed by the lane's 32-bit column offset
-  in.area = (a.area + x.orow)[sc.lo_q];
+  in.area = (a.area + sc.tg + x.orow)[sc.lo_q];
   in.cyu = (sc.cyk + (long long)(x.fc - a.js) * nid)[sc.lo_q];
   const long long frow = (long long)(x.fc - a.jsd) * nid;
   const long long gxrow = (long long)(x.rc - a.jsd) * (nid + 1) + (a.is - a.isd);
   const long long urow = x.orow + (a.is - a.isd);
   if constexpr (fp64) {
-    in.ya = (a.dya + frow - nid)[sc.lo_q];
-    in.yb = (a.sin_sg + 3 * a.ms2 + frow - nid)[sc.lo_q];
-    in.yc = (a.dya + frow)[sc.lo_q];
-    in.yd = (a.sin_sg + 1 * a.ms2 + frow)[sc.lo_q];
-    in.ye = (a.dx + frow)[sc.lo_q];
-    in.xa = (a.dxa + urow - 1)[sc.lo_fx];
-    in.xb = (a.sin_sg + 2 * a.ms2 + urow - 1)[sc.lo_fx];
-    in.xc = (a.dxa + urow)[sc.lo_fx];
-    in.xd = (a.sin_sg + 0 * a.ms2 + urow)[sc.lo_fx];
-    in.xe = (a.dy + gxrow)[sc.lo_fx];
+    in.ya = (a.dya + sc.tg + frow - nid)[sc.lo_q];
+    in.yb = (a.sin_sg + 9 * sc.tg + 3 * a.ms2 + frow - nid)[sc.lo_q];
+    in.yc = (a.dya + sc.tg + frow)[sc.lo_q];
+    in.yd = (a.sin_sg + 9 * sc.tg + 1 * a.ms2 + frow)[sc.lo_q];
+    in.ye = (a.dx + sc.tdx + frow)[sc.lo_q];
+    in.xa = (a.dxa + sc.tg + urow - 1)[sc.lo_fx];
+    in.xb = (a.sin_sg + 9 * sc.tg + 2 * a.ms2 + urow - 1)[sc.lo_fx];
+    in.xc = (a.dxa + sc.tg + urow)[sc.lo_fx];
+    in.xd = (a.sin_sg + 9 * sc.tg + 0 * a.ms2 + urow)[sc.lo_fx];
+    in.xe = (a.dy + sc.tdy + gxrow)[sc.lo_fx];
   } else {
-    in.ya = (a.gyp + frow)[sc.lo_q];
-    in.yb = (a.gym + frow)[sc.lo_q];
-    in.xa = (a.gxp + gxrow)[sc.lo_fx];
-    in.xb = (a.gxm + gxrow)[sc.lo_fx];
+    in.ya = (a.gyp + sc.tgf + frow)[sc.lo_q];
+    in.yb = (a.gym + sc.tgf + frow)[sc.lo_q];
+    in.xa = (a.gxp + sc.tgf + gxrow)[sc.lo_fx];
+    in.xb = (a.gxm + sc.tgf + gxrow)[sc.lo_fx];
   }
   in.cxu = (sc.cxk + (long long)(x.rc - a.jsd) * sc.nix)[sc.lo_fx];
   in.mfy = (sc.mfyk + (long long)(x.fc - a.js) * sc.niy)[sc.lo_mx];
@@ -115,7 +116,7 @@ __device__ __forceinline__ void row_fetch(const FusedArgs<T> &a, const StripCtx<
   in.rarea = T(0);
   in.dp1 = 0.0;
   if (x.out_row) {
-    in.rarea = (a.rarea + x.corow)[sc.lo_q];
+    in.rarea = (a.rarea + sc.tg + x.corow)[sc.lo_q];
     in.dp1 = (sc.dp1k + x.corow)[sc.lo_q];
   }
 #pragma unroll
@@ -216,8 +217,8 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   const T mfxp = from_right(mfxs);
   T dxa_m = T(1), dxa_r = T(1);
   if (sc.edge_strip) {
-    dxa_m = a.dxa[(long long)(x.mc - a.jsd) * nid + sc.lo_q];
-    dxa_r = a.dxa[x.orow + sc.lo_q];
+    dxa_m = (a.dxa + sc.tg)[(long long)(x.mc - a.jsd) * nid + sc.lo_q];
+    dxa_r = (a.dxa + sc.tg)[x.orow + sc.lo_q];
   }
   // cube-edge role of cell r-2 along the columns (uniform across the wave)
   const int role_y = sc.edge_rows ? edge_role(r - 2, a.js, a.je, a.npy, a.edge) : 0;
@@ -225,7 +226,7 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   if (role_y != 0) {
     auto dya_at = [&](int jj) {
       jj = jj < a.jsd ? a.jsd : (jj > a.jed ? a.jed : jj);
-      return a.dya[(long long)(jj - a.jsd) * nid + sc.lo_q];
+      return (a.dya + sc.tg)[(long long)(jj - a.jsd) * nid + sc.lo_q];
     };
     dd0 = dya_at(r - 4);
     dd1 = dya_at(r - 3);
@@ -291,34 +292,13 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
 // EDGE = false: the lean variant for the interior blocks (no cube-edge,
 // corner or halo-lane code is compiled); EDGE = true: every other block.
 template <typename T, int W, int NT, int HO, bool EDGE>
-__global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1) ? 1 : 3) k_fused_tp(FusedArgs<T> a) {
+__global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1 || sizeof(T) == 8) ? 1 : 3) k_fused_tp(const FusedArgs<T> a) {
   const int lane = threadIdx.x & (FT_LANES - 1), wave = threadIdx.x / FT_LANES;
   const int k = blockIdx.y;
   const int nspl = a.nsplit[k];
   // whole block: this level is done, or needs more passes than are launched
   // (reported by the caller's nsplit check; nothing is written for it)
   if (a.it > nspl || nspl > a.cap) return;
-  if (a.ntiles > 1) {  // this block's tile: offset every per-tile array
-    const long long t = blockIdx.z;
-    a.cx += t * a.tcx;
-    a.cy += t * a.tcy;
-    a.mfx += t * a.tmx;
-    a.mfy += t * a.tmy;
-    a.dp1 += t * a.tdp;
-    a.area += t * a.tgs2;
-    a.rarea += t * a.tgs2;
-    a.dxa += t * a.tgs2;
-    a.dya += t * a.tgs2;
-    a.sin_sg += t * 9 * a.tgs2;
-    a.dx += t * a.tgdx;
-    a.dy += t * a.tgdy;
-    if (sizeof(T) == 4) {
-      a.gxp += t * a.tgf;
-      a.gxm += t * a.tgf;
-      a.gyp += t * a.tgf;
-      a.gym += t * a.tgf;
-    }
-  }
   int strip, vb;
   if constexpr (EDGE) {
     strip = blockIdx.x % a.nstrip;
@@ -347,11 +327,18 @@ __global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1) ? 1 : 3) k_fused
   sc.frac = T(1) / T(nspl);
   sc.nix = a.ie - a.is + 2;
   sc.niy = a.ie - a.is + 1;
-  sc.cxk = a.cx + (long long)k * a.sx;
-  sc.cyk = a.cy + (long long)k * a.sy;
-  sc.mfxk = a.mfx + (long long)k * a.smx;
-  sc.mfyk = a.mfy + (long long)k * a.smy;
-  sc.dp1k = a.dp1 + (long long)k * a.s2;
+  // this block's tile (the argument block itself stays unmodified: a kernel
+  // argument written to is copied to the private stack)
+  const long long tile = blockIdx.z;
+  sc.tg = tile * a.tgs2;
+  sc.tdx = tile * a.tgdx;
+  sc.tdy = tile * a.tgdy;
+  sc.tgf = tile * a.tgf;
+  sc.cxk = a.cx + tile * a.tcx + (long long)k * a.sx;
+  sc.cyk = a.cy + tile * a.tcy + (long long)k * a.sy;
+  sc.mfxk = a.mfx + tile * a.tmx + (long long)k * a.smx;
+  sc.mfyk = a.mfy + tile * a.tmy + (long long)k * a.smy;
+  sc.dp1k = a.dp1 + tile * a.tdp + (long long)k * a.s2;
   const T *in;
   T *out;
   pass_buffers(a, nspl, in, out);

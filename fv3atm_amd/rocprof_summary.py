@@ -1,0 +1,119 @@
+"""Short text summaries of rocprofv3 CSV output (``--output-format csv``).
+
+  python -m fv3atm_amd.rocprof_summary stats DIR
+      per-kernel calls, total and mean duration from the kernel trace
+  python -m fv3atm_amd.rocprof_summary counters DIR [--calib-bytes B]
+      per-kernel mean of every collected counter per dispatch; with
+      --calib-bytes, the counter of the calibration kernel (the one whose
+      name contains --calib-name, default "elementwise") is taken to stand
+      for B bytes, and every other kernel's counter is scaled by the same
+      factor into HBM bytes per dispatch.
+
+The summaries print the figures only; the rocprofv3 command that made DIR
+is recorded by the caller beside them.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import os
+import re
+import sys
+from collections import defaultdict
+
+
+def _rows(d, suffix):
+    out = []
+    for p in sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)):
+        with open(p, newline="") as f:
+            out.extend(csv.DictReader(f))
+    if not out:
+        raise SystemExit(f"no *{suffix} under {d}")
+    return out
+
+
+def _col(row, *names):
+    low = {k.lower(): k for k in row}
+    for n in names:
+        if n.lower() in low:
+            return row[low[n.lower()]]
+    raise KeyError(f"none of {names} in {list(row)}")
+
+
+def short(name):
+    """Kernel name without its argument list, keeping template arguments."""
+    s = re.sub(r"^void\s+", "", name.strip())
+    depth, cut = 0, len(s)
+    for i, ch in enumerate(s):  # first '(' outside template brackets
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    return s[:cut]
+
+
+def kernel_stats(d):
+    agg = defaultdict(lambda: [0, 0.0])
+    for r in _rows(d, "kernel_trace.csv"):
+        n = short(_col(r, "Kernel_Name"))
+        t = float(_col(r, "End_Timestamp")) - float(_col(r, "Start_Timestamp"))
+        agg[n][0] += 1
+        agg[n][1] += t
+    return agg
+
+
+def counter_means(d):
+    """{(kernel, counter): [dispatches, sum]} over dispatches (a counter that a
+    dispatch reports once per agent/dimension is summed within the dispatch)."""
+    per = defaultdict(float)
+    names = {}
+    for r in _rows(d, "counter_collection.csv"):
+        key = (_col(r, "Dispatch_Id"), _col(r, "Counter_Name"))
+        per[key] += float(_col(r, "Counter_Value"))
+        names[key[0]] = short(_col(r, "Kernel_Name"))
+    agg = defaultdict(lambda: [0, 0.0])
+    for (disp, cn), v in per.items():
+        a = agg[(names[disp], cn)]
+        a[0] += 1
+        a[1] += v
+    return agg
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=["stats", "counters"])
+    ap.add_argument("dir")
+    ap.add_argument("--calib-bytes", type=float, default=0.0)
+    ap.add_argument("--calib-name", default="elementwise")
+    ap.add_argument("--top", type=int, default=12)
+    a = ap.parse_args(argv)
+    w = sys.stdout.write
+    if a.mode == "stats":
+        agg = kernel_stats(a.dir)
+        tot = sum(v[1] for v in agg.values())
+        w("| kernel | calls | total ms | mean us | share |\n|---|---:|---:|---:|---:|\n")
+        for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[: a.top]:
+            w(f"| `{n}` | {c} | {t / 1e6:.3f} | {t / c / 1e3:.1f} | {100 * t / tot:.1f}% |\n")
+        return
+    agg = counter_means(a.dir)
+    factor = {}
+    if a.calib_bytes > 0:
+        for (n, cn), (c, s) in agg.items():
+            if a.calib_name in n:
+                factor[cn] = a.calib_bytes / (s / c * 1024.0)  # counters read KiB
+                w(f"calibration `{n}` {cn}: {s / c:.0f} KiB per dispatch for {a.calib_bytes:.0f} B "
+                  f"-> factor {factor[cn]:.3f}\n")
+    w("| kernel | counter | dispatches | mean KiB/dispatch | HBM bytes/dispatch |\n"
+      "|---|---|---:|---:|---:|\n")
+    for (n, cn), (c, s) in sorted(agg.items()):
+        m = s / c
+        hb = f"{m * 1024.0 * factor[cn]:.4e}" if cn in factor else "-"
+        w(f"| `{n}` | {cn} | {c} | {m:.1f} | {hb} |\n")
+
+
+if __name__ == "__main__":
+    main()
