@@ -11,20 +11,21 @@
 namespace fv3 {
 
 template <int HO>
-static void go64(const FusedArgs<double> &a, int nblk, hipStream_t s) {
+static void go64(const FusedArgs<double> &a, hipStream_t s) {
   const dim3 b(FT_LANES * 4);
-  const int nin = a.s_hi >= a.s_lo && a.v_hi >= a.v_lo ? (a.s_hi - a.s_lo + 1) * (a.v_hi - a.v_lo + 1) : 0;
-  if (nin > 0)
-    hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, false>), dim3((unsigned)nin, (unsigned)a.npz, (unsigned)a.ntiles), b, 0, s, a);
-  if (nin < nblk)
-    hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, true>), dim3((unsigned)nblk, (unsigned)a.npz, (unsigned)a.ntiles), b, 0, s, a);
+  int n0, n1, n2;
+  fused_grid_split(a, n0, n1, n2);
+  auto grid = [&](int n) { return dim3((unsigned)n, (unsigned)a.npz, (unsigned)a.ntiles); };
+  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 0, 1>), grid(n0), b, 0, s, a);
+  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 1, 1>), grid(n1), b, 0, s, a);
+  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 2, 1>), grid(n2), b, 0, s, a);
 }
 
-int launch_fused_f64(const FusedArgs<double> &a, int nblk, hipStream_t s) {
+int launch_fused_f64(const FusedArgs<double> &a, hipStream_t s) {
   if (a.hord == 8)
-    go64<8>(a, nblk, s);
+    go64<8>(a, s);
   else
-    go64<0>(a, nblk, s);
+    go64<0>(a, s);
   FV3_HIP(hipGetLastError());
   return 0;
 }
@@ -151,16 +152,15 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     a.gym = a.gyp + (long long)d.nid * (d.njd + 1);
     a.tgf = 2 * (long long)(d.nid + 1) * d.njd + 2 * (long long)d.nid * (d.njd + 1);
   }
-  const int nblk = a.nstrip * a.nvblk;
   const bool timing = c.timing && !c.graph_safe;
   if (timing) FV3_HIP(hipEventRecord(c.ev[0], s));
   for (int it = 1; it <= cap; it++) {
     a.it = it;
     int rc;
     if constexpr (std::is_same<T, double>::value)
-      rc = launch_fused_f64(a, nblk, s);
+      rc = launch_fused_f64(a, s);
     else
-      rc = launch_fused_f32(a, nblk, c.fused_cfg, s);
+      rc = launch_fused_f32(a, c.fused_cfg, s);
     if (rc) return rc;
   }
   if (nsplit_cap > 0) {

@@ -5,36 +5,42 @@
 
 namespace fv3 {
 
-template <int W, int NT, int HO>
-static void go2(const FusedArgs<float> &a, int nblk, hipStream_t s) {
+// W waves per workgroup x NT tracers per wave; OCC_E: occupancy asked of the
+// edge kinds' register allocation (the lean kind with one tracer per wave
+// fits three waves per SIMD without spilling)
+template <int W, int NT, int HO, int OCC_E>
+static void go2(const FusedArgs<float> &a, hipStream_t s) {
   const dim3 b(FT_LANES * W);
-  const int nin = a.s_hi >= a.s_lo && a.v_hi >= a.v_lo ? (a.s_hi - a.s_lo + 1) * (a.v_hi - a.v_lo + 1) : 0;
-  if (nin > 0)
-    hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, false>), dim3((unsigned)nin, (unsigned)a.npz, (unsigned)a.ntiles), b, 0, s, a);
-  if (nin < nblk)
-    hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, true>), dim3((unsigned)nblk, (unsigned)a.npz, (unsigned)a.ntiles), b, 0, s, a);
+  constexpr int OCC_L = NT > 1 ? 1 : (W >= 4 ? 12 / W : 3);
+  int n0, n1, n2;
+  fused_grid_split(a, n0, n1, n2);
+  auto grid = [&](int n) { return dim3((unsigned)n, (unsigned)a.npz, (unsigned)a.ntiles); };
+  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 0, OCC_L>), grid(n0), b, 0, s, a);
+  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 1, OCC_E>), grid(n1), b, 0, s, a);
+  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 2, OCC_E>), grid(n2), b, 0, s, a);
 }
 
-template <int W, int NT>
-static int go(const FusedArgs<float> &a, int nblk, hipStream_t s) {
+template <int W, int NT, int OCC_E = 1>
+static int go(const FusedArgs<float> &a, hipStream_t s) {
   if (a.hord == 8)  // the production order, specialised at compile time
-    go2<W, NT, 8>(a, nblk, s);
+    go2<W, NT, 8, OCC_E>(a, s);
   else
-    go2<W, NT, 0>(a, nblk, s);
+    go2<W, NT, 0, OCC_E>(a, s);
   FV3_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_fused_f32(const FusedArgs<float> &a, int nblk, int cfg, hipStream_t s) {
+int launch_fused_f32(const FusedArgs<float> &a, int cfg, hipStream_t s) {
   switch (cfg) {
-    case 0: return go<4, 1>(a, nblk, s);
-    case 1: return go<8, 1>(a, nblk, s);
-    case 2: return go<4, 2>(a, nblk, s);
-    case 3: return go<2, 2>(a, nblk, s);
-    case 4: return go<6, 1>(a, nblk, s);
-    case 5: return go<1, 2>(a, nblk, s);
-    case 6: return go<1, 1>(a, nblk, s);
-    case 7: return go<2, 1>(a, nblk, s);
+    case 0: return go<4, 1>(a, s);
+    case 1: return go<8, 1>(a, s);
+    case 2: return go<4, 2>(a, s);
+    case 3: return go<2, 2>(a, s);
+    case 4: return go<6, 1>(a, s);
+    case 5: return go<1, 2>(a, s);
+    case 6: return go<1, 1>(a, s);
+    case 7: return go<2, 1>(a, s);
+    case 8: return go<4, 1, 3>(a, s);
     default: return set_error("unknown fused FP32 launch configuration %d", cfg);
   }
 }

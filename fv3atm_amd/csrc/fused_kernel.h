@@ -80,6 +80,56 @@ __device__ __forceinline__ RowIdx<T> row_idx(const FusedArgs<T> &a, const StripC
   return x;
 }
 
+// Lane source of the pass input q at input row x for the y-sweep (py) and
+// the x-sweep (px): the cell itself inside the compute domain; on a frame
+// block, halo cells of one tile come from the caller's halo, halo cells of
+// six tiles from the neighbour face, and corner cells from their
+// copy_corners source (dir 2 for the y-sweep, dir 1 for the x-sweep).
+template <typename T, int EK>
+__device__ __forceinline__ void q_src(const FusedArgs<T> &a, const StripCtx<T> &sc, const TrState<T> &s,
+                                      const RowIdx<T> &x, const T *&py, const T *&px) {
+  const int nid = a.nid;
+  // EK 1 blocks have every lane inside the compute domain, EK 0 every cell
+  const bool inside = EK == 0 || ((EK == 1 || sc.lane_in) && x.row_in);
+  py = (inside ? s.qin : s.qhalo) + x.orow + sc.lo_q;
+  px = py;
+  if constexpr (EK == 1) {
+    if (a.ntiles > 1 && !inside) {  // edge halo rows of a cube face (no corner cell)
+      int ts, si, sj;
+      cube_halo_src(blockIdx.z + 1, sc.ic, x.rc, a.ie, ts, si, sj);
+      py = s.qsrc0 + (long long)(ts - 1) * a.tq + (long long)(sj - a.jsd) * nid + (si - a.isd);
+      px = py;
+    }
+  }
+  if constexpr (EK == 2) {
+    if (a.ntiles > 1 && !inside) {
+      const int n = a.ie;
+      const int tile = blockIdx.z + 1;
+      auto from_cube = [&](int ci, int cj) -> const T * {
+        int ts, si, sj;
+        cube_halo_src(tile, ci, cj, n, ts, si, sj);
+        return s.qsrc0 + (long long)(ts - 1) * a.tq + (long long)(sj - a.jsd) * nid + (si - a.isd);
+      };
+      int si, sj;
+      if (corner_src(sc.ic, x.rc, a.npx, a.npy, 2, a.corners, si, sj)) {
+        py = from_cube(si, sj);
+        corner_src(sc.ic, x.rc, a.npx, a.npy, 1, a.corners, si, sj);
+        px = from_cube(si, sj);
+      } else {
+        py = from_cube(sc.ic, x.rc);
+        px = py;
+      }
+    } else if (sc.corner_wg && !x.row_in && !sc.lane_in) {
+      int si, sj;
+      if (corner_src(sc.ic, x.rc, a.npx, a.npy, 2, a.corners, si, sj)) {
+        py = s.qhalo + (long long)(sj - a.jsd) * nid + (si - a.isd);
+        corner_src(sc.ic, x.rc, a.npx, a.npy, 1, a.corners, si, sj);
+        px = s.qhalo + (long long)(sj - a.jsd) * nid + (si - a.isd);
+      }
+    }
+  }
+}
+
 template <typename T, int NT>
 __device__ __forceinline__ void row_fetch(const FusedArgs<T> &a, const StripCtx<T> &sc,
                                           const TrState<T> (&st)[NT], const bool (&act)[NT], int t,
@@ -122,39 +172,13 @@ __device__ __forceinline__ void row_fetch(const FusedArgs<T> &a, const StripCtx<
 #pragma unroll
   for (int n = 0; n < NT; n++) {
     if (!act[n]) continue;
-    const TrState<T> &s = st[n];
-    // lean blocks have no halo lanes or rows: the pass input is always read
-    T qy = (!sc.edge_rows || (sc.lane_in && x.row_in)) ? (s.qin + x.orow)[sc.lo_q]
-                                                       : (s.qhalo + x.orow)[sc.lo_q];
-    T qx = qy;
-    if (sc.edge_rows && a.ntiles > 1 && !(sc.lane_in && x.row_in)) {
-      // six tiles: edge-halo cells come from the neighbour face; corner
-      // cells take their copy_corners source first (dir 2 for the y-sweep,
-      // dir 1 for the x-sweep), which is an edge-halo cell
-      const int n = a.ie;
-      const int tile = blockIdx.z + 1;
-      auto from_cube = [&](int ci, int cj) -> T {
-        int ts, si, sj;
-        cube_halo_src(tile, ci, cj, n, ts, si, sj);
-        return (s.qsrc0 + (long long)(ts - 1) * a.tq)[(long long)(sj - a.jsd) * nid + (si - a.isd)];
-      };
-      int si, sj;
-      if (corner_src(sc.ic, x.rc, a.npx, a.npy, 2, a.corners, si, sj)) {
-        qy = from_cube(si, sj);
-        corner_src(sc.ic, x.rc, a.npx, a.npy, 1, a.corners, si, sj);
-        qx = from_cube(si, sj);
-      } else {
-        qy = from_cube(sc.ic, x.rc);
-        qx = qy;
-      }
-    } else if (sc.corner_wg && !x.row_in && !sc.lane_in) {
-      int si, sj;
-      if (corner_src(sc.ic, x.rc, a.npx, a.npy, 2, a.corners, si, sj)) {
-        qy = s.qhalo[(long long)(sj - a.jsd) * nid + (si - a.isd)];
-        corner_src(sc.ic, x.rc, a.npx, a.npy, 1, a.corners, si, sj);
-        qx = s.qhalo[(long long)(sj - a.jsd) * nid + (si - a.isd)];
-      }
-    }
+    const T *py, *px;
+    if (sc.edge_rows)
+      q_src<T, 2>(a, sc, st[n], x, py, px);
+    else
+      q_src<T, 0>(a, sc, st[n], x, py, px);
+    const T qy = *py;
+    const T qx = sc.edge_rows ? *px : qy;
     in.qy[n] = qy;
     in.qx[n] = qx;
   }
@@ -289,10 +313,117 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   lv.mfy_prev = mfys;
 }
 
-// EDGE = false: the lean variant for the interior blocks (no cube-edge,
-// corner or halo-lane code is compiled); EDGE = true: every other block.
-template <typename T, int W, int NT, int HO, bool EDGE>
-__global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1 || sizeof(T) == 8) ? 1 : 3) k_fused_tp(const FusedArgs<T> a) {
+// ---- LDS row ring of the lean FP32 kernel.  Rows are loaded straight into
+// LDS with global_load_lds_dword (the data in flight holds no VGPRs), so the
+// loads of a row can be issued LR_DEPTH-1 rows ahead.  The waves of a
+// workgroup split the LR_NL level arrays of a row between them (they stream
+// the same strip and level) and each loads its own tracers' rows.
+constexpr int LR_DEPTH = 4;
+constexpr int LR_NL = 12;  // area cyu cxu mfy mfx rarea gyp gym gxp gxm dp1.lo dp1.hi
+
+template <int W, int NT, int QN>
+struct LdsRing {
+  float v[LR_DEPTH][LR_NL + W * NT * QN][FT_LANES];
+};
+
+__device__ __forceinline__ void lds_dma(const float *src, float *dst_row) {
+  __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)dst_row, 4, 0, 0);
+}
+
+// s_waitcnt vmcnt(n) lgkmcnt(0) (gfx9 encoding: vmcnt in bits 3:0 and 15:14,
+// expcnt 6:4 left at its no-wait maximum, lgkmcnt 11:8)
+template <int N>
+__device__ __forceinline__ void wait_vm_lgkm0() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | (0x7 << 4));
+}
+
+// issue the loads of input row t into ring slot t % LR_DEPTH
+template <int W, int NT, int EK>
+__device__ __forceinline__ void ring_issue(const FusedArgs<float> &a, const StripCtx<float> &sc,
+                                           const TrState<float> (&st)[NT],
+                                           LdsRing<W, NT, EK == 2 ? 2 : 1> &ring, int wave, int t) {
+  constexpr int QN = EK == 2 ? 2 : 1;
+  const RowIdx<float> x = row_idx(a, sc, t);
+  const int nid = a.nid;
+  float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
+  const long long frow = (long long)(x.fc - a.jsd) * nid;
+  const long long gxrow = (long long)(x.rc - a.jsd) * (nid + 1) + (a.is - a.isd);
+  for (int j = wave; j < LR_NL; j += W) {
+    const float *p;
+    switch (j) {
+      case 0: p = a.area + sc.tg + x.orow + sc.lo_q; break;
+      case 1: p = sc.cyk + (long long)(x.fc - a.js) * nid + sc.lo_q; break;
+      case 2: p = sc.cxk + (long long)(x.rc - a.jsd) * sc.nix + sc.lo_fx; break;
+      case 3: p = sc.mfyk + (long long)(x.fc - a.js) * sc.niy + sc.lo_mx; break;
+      case 4: p = sc.mfxk + (long long)(x.mc - a.js) * sc.nix + sc.lo_fx; break;
+      case 5: p = a.rarea + sc.tg + x.corow + sc.lo_q; break;
+      case 6: p = a.gyp + sc.tgf + frow + sc.lo_q; break;
+      case 7: p = a.gym + sc.tgf + frow + sc.lo_q; break;
+      case 8: p = a.gxp + sc.tgf + gxrow + sc.lo_fx; break;
+      case 9: p = a.gxm + sc.tgf + gxrow + sc.lo_fx; break;
+      case 10: p = (const float *)(sc.dp1k + x.corow + sc.lo_q); break;
+      default: p = (const float *)(sc.dp1k + x.corow + sc.lo_q) + 1; break;
+    }
+    lds_dma(p, slot[j]);
+  }
+  // inactive tracers load tracer 0: every wave issues the same count
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    const float *py, *px;
+    q_src<float, EK>(a, sc, st[n], x, py, px);
+    lds_dma(py, slot[LR_NL + (wave * NT + n) * QN]);
+    if (QN == 2) lds_dma(px, slot[LR_NL + (wave * NT + n) * QN + 1]);
+  }
+}
+
+template <int W, int NT, int QN>
+__device__ __forceinline__ void ring_read(const LdsRing<W, NT, QN> &ring, int wave, int lane, int t,
+                                          RowIn<float, NT> &in) {
+  const float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
+  in.area = slot[0][lane];
+  in.cyu = slot[1][lane];
+  in.cxu = slot[2][lane];
+  in.mfy = slot[3][lane];
+  in.mfx = slot[4][lane];
+  in.rarea = slot[5][lane];
+  in.ya = slot[6][lane];
+  in.yb = slot[7][lane];
+  in.xa = slot[8][lane];
+  in.xb = slot[9][lane];
+  const unsigned long long lo = __float_as_uint(slot[10][lane]), hi = __float_as_uint(slot[11][lane]);
+  in.dp1 = __longlong_as_double((long long)(lo | (hi << 32)));
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    in.qy[n] = slot[LR_NL + (wave * NT + n) * QN][lane];
+    in.qx[n] = QN == 2 ? slot[LR_NL + (wave * NT + n) * QN + 1][lane] : in.qy[n];
+  }
+}
+
+// wait until this wave's loads of row t are in LDS (loads complete in issue
+// order; ahead = rows issued after t; every wave issues at least NMIN loads
+// per row), then until every wave's are (barrier)
+template <int W, int NT, int QN>
+__device__ __forceinline__ void ring_wait(int ahead) {
+  constexpr int NMIN = LR_NL / W + NT * QN;
+  static_assert((LR_DEPTH - 2) * NMIN < 64, "vmcnt range");
+  if (ahead >= 2)
+    wait_vm_lgkm0<2 * NMIN>();
+  else if (ahead == 1)
+    wait_vm_lgkm0<NMIN>();
+  else
+    wait_vm_lgkm0<0>();
+  if (W > 1) __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");  // no LDS access moves across the barrier
+}
+
+// EK (edge kind) 0: the lean variant for the interior blocks (no cube-edge,
+// corner or halo code is compiled); 1: blocks of the interior strips that
+// reach a halo or cube-edge row (no per-lane edge or corner code); 2: the
+// strips that reach a halo or cube-edge column.
+// OCC: minimum workgroups per CU asked of the register allocator.
+template <typename T, int W, int NT, int HO, int EK, int OCC>
+__global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T> a) {
   const int lane = threadIdx.x & (FT_LANES - 1), wave = threadIdx.x / FT_LANES;
   const int k = blockIdx.y;
   const int nspl = a.nsplit[k];
@@ -300,14 +431,23 @@ __global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1 || sizeof(T) == 8
   // (reported by the caller's nsplit check; nothing is written for it)
   if (a.it > nspl || nspl > a.cap) return;
   int strip, vb;
-  if constexpr (EDGE) {
-    strip = blockIdx.x % a.nstrip;
-    vb = blockIdx.x / a.nstrip;
-    if (strip >= a.s_lo && strip <= a.s_hi && vb >= a.v_lo && vb <= a.v_hi) return;  // lean variant's
-  } else {
-    const int ns = a.s_hi - a.s_lo + 1;
-    strip = a.s_lo + (int)(blockIdx.x % ns);
-    vb = a.v_lo + (int)(blockIdx.x / ns);
+  {
+    const int ns = a.s_hi >= a.s_lo ? a.s_hi - a.s_lo + 1 : 0;
+    const int nv = a.v_hi >= a.v_lo ? a.v_hi - a.v_lo + 1 : 0;
+    const int bx = (int)blockIdx.x;
+    if constexpr (EK == 0) {
+      strip = a.s_lo + bx % ns;
+      vb = a.v_lo + bx / ns;
+    } else if constexpr (EK == 1) {  // interior strips, row blocks outside [v_lo, v_hi]
+      strip = a.s_lo + bx % ns;
+      vb = bx / ns;
+      if (vb >= a.v_lo) vb += nv;
+    } else {  // strips outside [s_lo, s_hi], every row block
+      const int nso = a.nstrip - ns;
+      strip = bx % nso;
+      if (ns > 0 && strip >= a.s_lo) strip += ns;
+      vb = bx / nso;
+    }
   }
   const int i0 = a.is + strip * FT_BX;
   StripCtx<T> sc;
@@ -320,9 +460,9 @@ __global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1 || sizeof(T) == 8
   sc.lo_mx = (sc.i > a.ie ? a.ie : (sc.i < a.is ? a.is : sc.i)) - a.is;
   sc.col_out = lane >= NG && lane < NG + FT_BX && sc.i <= a.ie;
   sc.lane_in = sc.i >= a.is && sc.i <= a.ie;
-  sc.edge_strip = EDGE && a.edge && (i0 - NG <= 2 || i0 - NG + FT_LANES - 1 >= a.npx - 2);
-  sc.edge_rows = EDGE;
-  sc.corner_wg = EDGE && a.corners != 0u && (strip == 0 || strip == a.nstrip - 1) && (vb == 0 || vb == a.nvblk - 1);
+  sc.edge_strip = EK == 2 && a.edge && (i0 - NG <= 2 || i0 - NG + FT_LANES - 1 >= a.npx - 2);
+  sc.edge_rows = EK != 0;
+  sc.corner_wg = EK == 2 && a.corners != 0u && (strip == 0 || strip == a.nstrip - 1) && (vb == 0 || vb == a.nvblk - 1);
   sc.scale = nspl > 1;
   sc.frac = T(1) / T(nspl);
   sc.nix = a.ie - a.is + 2;
@@ -363,6 +503,31 @@ __global__ void __launch_bounds__(FT_LANES *W, (EDGE || NT > 1 || sizeof(T) == 8
     LvState<T> lv;
     lv.yfx_prev = lv.mfy_prev = T(0);
     const bool any = act[0];
+    if constexpr (sizeof(T) == 4) {
+      constexpr int QN = EK == 2 ? 2 : 1;
+      static_assert(LR_DEPTH == 4, "ring_wait covers up to two rows ahead");
+      __shared__ LdsRing<W, NT, QN> ring;
+      // the previous round's last reads of every slot are done before the
+      // prologue overwrites them
+      if (rd > 0) ring_wait<W, NT, QN>(0);
+      for (int t = 0; t < LR_DEPTH - 1 && t < nrow; t++) ring_issue<W, NT, EK>(a, sc, st, ring, wave, t);
+      for (int t0 = 0; t0 < nrow; t0 += 6) {
+#define FV3_RROW(U)                                                                  \
+  if (t0 + U < nrow) {                                                               \
+    const int t = t0 + U;                                                            \
+    ring_wait<W, NT, QN>(min(LR_DEPTH - 2, nrow - 1 - t));                           \
+    if (t + LR_DEPTH - 1 < nrow) ring_issue<W, NT, EK>(a, sc, st, ring, wave, t + LR_DEPTH - 1); \
+    if (any) {                                                                       \
+      RowIn<T, NT> in;                                                               \
+      ring_read<W, NT, QN>(ring, wave, lane, t, in);                                 \
+      row_step<T, NT, U, HO>(a, sc, lv, st, act, t, in);                             \
+    }                                                                                \
+  }
+        FV3_RROW(0) FV3_RROW(1) FV3_RROW(2) FV3_RROW(3) FV3_RROW(4) FV3_RROW(5)
+#undef FV3_RROW
+      }
+      continue;
+    }
     // rows are fetched one ahead; the bare s_barrier keeps the waves of the
     // workgroup (same level data, L1 reuse) in step without waiting for the
     // loads in flight -- the waves share no memory

@@ -62,6 +62,17 @@ struct FusedArgs {
   long long ms2;                   // metric slab (isd:ied, jsd:jed)
 };
 
+// Block counts of the three kernel kinds over (strip, row block): lean
+// interior blocks, edge-row blocks of the interior strips, edge strips.
+template <typename T>
+inline void fused_grid_split(const FusedArgs<T> &a, int &n_lean, int &n_rows, int &n_strips) {
+  const int ns = a.s_hi >= a.s_lo ? a.s_hi - a.s_lo + 1 : 0;
+  const int nv = a.v_hi >= a.v_lo ? a.v_hi - a.v_lo + 1 : 0;
+  n_lean = ns * nv;
+  n_rows = ns * (a.nvblk - nv);
+  n_strips = (a.nstrip - ns) * a.nvblk;
+}
+
 // ---- cross-lane shifts: value of lane l-1 / l+1 (DPP wave_shr:1 / wave_shl:1)
 // (bound_ctrl: the lane past the wave edge reads 0, so no destination
 // initialisation is needed; that lane is a halo lane whose result is unused)

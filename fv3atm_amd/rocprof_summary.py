@@ -5,7 +5,7 @@
   python -m fv3atm_amd.rocprof_summary counters DIR [--calib-bytes B]
       per-kernel mean of every collected counter per dispatch; with
       --calib-bytes, the counter of the calibration kernel (the one whose
-      name contains --calib-name, default "elementwise") is taken to stand
+      name contains --calib-name, default "bitwise_not") is taken to stand
       for B bytes, and every other kernel's counter is scaled by the same
       factor into HBM bytes per dispatch.
 
@@ -43,7 +43,7 @@ def _col(row, *names):
 
 def short(name):
     """Kernel name without its argument list, keeping template arguments."""
-    s = re.sub(r"^void\s+", "", name.strip())
+    s = re.sub(r"^void\s+", "", name.strip()).replace("(anonymous namespace)", "{anon}")
     depth, cut = 0, len(s)
     for i, ch in enumerate(s):  # first '(' outside template brackets
         if ch == "<":
@@ -54,6 +54,10 @@ def short(name):
             cut = i
             break
     return s[:cut]
+
+
+def clip(name, n=96):
+    return name if len(name) <= n else name[: n - 3] + "..."
 
 
 def kernel_stats(d):
@@ -88,7 +92,7 @@ def main(argv=None):
     ap.add_argument("mode", choices=["stats", "counters"])
     ap.add_argument("dir")
     ap.add_argument("--calib-bytes", type=float, default=0.0)
-    ap.add_argument("--calib-name", default="elementwise")
+    ap.add_argument("--calib-name", default="bitwise_not")
     ap.add_argument("--top", type=int, default=12)
     a = ap.parse_args(argv)
     w = sys.stdout.write
@@ -97,13 +101,13 @@ def main(argv=None):
         tot = sum(v[1] for v in agg.values())
         w("| kernel | calls | total ms | mean us | share |\n|---|---:|---:|---:|---:|\n")
         for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[: a.top]:
-            w(f"| `{n}` | {c} | {t / 1e6:.3f} | {t / c / 1e3:.1f} | {100 * t / tot:.1f}% |\n")
+            w(f"| `{clip(n)}` | {c} | {t / 1e6:.3f} | {t / c / 1e3:.1f} | {100 * t / tot:.1f}% |\n")
         return
     agg = counter_means(a.dir)
     factor = {}
     if a.calib_bytes > 0:
         for (n, cn), (c, s) in agg.items():
-            if a.calib_name in n:
+            if a.calib_name.lower() in n.lower():
                 factor[cn] = a.calib_bytes / (s / c * 1024.0)  # counters read KiB
                 w(f"calibration `{n}` {cn}: {s / c:.0f} KiB per dispatch for {a.calib_bytes:.0f} B "
                   f"-> factor {factor[cn]:.3f}\n")
@@ -112,7 +116,7 @@ def main(argv=None):
     for (n, cn), (c, s) in sorted(agg.items()):
         m = s / c
         hb = f"{m * 1024.0 * factor[cn]:.4e}" if cn in factor else "-"
-        w(f"| `{n}` | {cn} | {c} | {m:.1f} | {hb} |\n")
+        w(f"| `{clip(n)}` | {cn} | {c} | {m:.1f} | {hb} |\n")
 
 
 if __name__ == "__main__":

@@ -29,6 +29,10 @@ def test_graph_replays_equal_eager(gpu_ready):
     q_in, q_out, pt = q0.clone(), torch.zeros_like(q0), pt0.clone()
     delp = torch.zeros_like(dp1)
     cap = 4
+    nid = case.n + 6
+
+    def inner(t):  # the step writes interior cells only; halo cells keep what they held
+        return t.view(case.nq, case.npz, nid, nid)[:, :, 3:-3, 3:-3]
 
     def step(stream=None):
         fa.step_dev(ctx, fa.FORM_FUSED32, case.nq, q_in, q_out, pt, dp1, delp, w["cx"], w["cy"], w["mfx"],
@@ -63,8 +67,9 @@ def test_graph_replays_equal_eager(gpu_ready):
         torch.cuda.synchronize()
         g.replay()
         torch.cuda.synchronize()
-        print(f"replay {rep}: max |dq| {(q_out - q_ref).abs().max().item():.3e}", float(junk) * 0, flush=True)
-        assert torch.equal(q_out, q_ref), f"replay {rep} differs"
+        dq = (inner(q_out) - inner(q_ref)).abs().max().item()
+        print(f"replay {rep}: max |dq| {dq:.3e}", float(junk) * 0, flush=True)
+        assert torch.equal(inner(q_out), inner(q_ref)), f"replay {rep} differs"
         assert torch.equal(pt, pt_ref) and torch.equal(delp, delp_ref)
         assert fa.step_status(ctx) == 0
     fa.set_graph_safe(ctx, False)
