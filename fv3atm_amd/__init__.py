@@ -210,4 +210,4 @@ def set_launch(ctx: Context, cfg: int = 0, rows: int = 0, remap: int = 0):
     check(_ffi.lib().fv3_ctx_set_launch(ctx.handle, cfg, rows, remap), "fv3_ctx_set_launch")
 
 
-N_LAUNCH_CFG = 9
+N_LAUNCH_CFG = 15

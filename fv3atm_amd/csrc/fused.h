@@ -8,7 +8,7 @@ namespace fv3 {
 
 // launch configurations (waves per workgroup x tracers per wave); cfg 0 is
 // the default, the others are the autotuner's candidates
-constexpr int FUSED32_NCFG = 9;
+constexpr int FUSED32_NCFG = 15;
 int launch_fused_f64(const FusedArgs<double> &a, hipStream_t s);
 int launch_fused_f32(const FusedArgs<float> &a, int cfg, hipStream_t s);
 

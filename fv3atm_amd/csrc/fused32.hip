@@ -5,27 +5,27 @@
 
 namespace fv3 {
 
-// W waves per workgroup x NT tracers per wave; OCC_E: occupancy asked of the
-// edge kinds' register allocation (the lean kind with one tracer per wave
-// fits three waves per SIMD without spilling)
-template <int W, int NT, int HO, int OCC_E>
+// W waves per workgroup x NT tracers per wave for the lean kind, WE x NTE
+// for the two edge kinds (separate launches over disjoint blocks); OCC_E:
+// occupancy asked of the edge kinds' register allocation (the lean kind with
+// one tracer per wave fits three waves per SIMD without spilling)
+template <int W, int NT, int HO, int OCC_E, int WE, int NTE, int OCC_L0>
 static void go2(const FusedArgs<float> &a, hipStream_t s) {
-  const dim3 b(FT_LANES * W);
-  constexpr int OCC_L = NT > 1 ? 1 : (W >= 4 ? 12 / W : 3);
+  constexpr int OCC_L = OCC_L0 > 0 ? OCC_L0 : (NT > 1 ? 1 : (W >= 4 ? 12 / W : 3));
   int n0, n1, n2;
   fused_grid_split(a, n0, n1, n2);
   auto grid = [&](int n) { return dim3((unsigned)n, (unsigned)a.npz, (unsigned)a.ntiles); };
-  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 0, OCC_L>), grid(n0), b, 0, s, a);
-  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 1, OCC_E>), grid(n1), b, 0, s, a);
-  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 2, OCC_E>), grid(n2), b, 0, s, a);
+  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 0, OCC_L>), grid(n0), dim3(FT_LANES * W), 0, s, a);
+  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 1, OCC_E>), grid(n1), dim3(FT_LANES * WE), 0, s, a);
+  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 2, OCC_E>), grid(n2), dim3(FT_LANES * WE), 0, s, a);
 }
 
-template <int W, int NT, int OCC_E = 1>
+template <int W, int NT, int OCC_E = 1, int WE = W, int NTE = NT, int OCC_L = 0>
 static int go(const FusedArgs<float> &a, hipStream_t s) {
   if (a.hord == 8)  // the production order, specialised at compile time
-    go2<W, NT, 8, OCC_E>(a, s);
+    go2<W, NT, 8, OCC_E, WE, NTE, OCC_L>(a, s);
   else
-    go2<W, NT, 0, OCC_E>(a, s);
+    go2<W, NT, 0, OCC_E, WE, NTE, OCC_L>(a, s);
   FV3_HIP(hipGetLastError());
   return 0;
 }
@@ -41,6 +41,12 @@ int launch_fused_f32(const FusedArgs<float> &a, int cfg, hipStream_t s) {
     case 6: return go<1, 1>(a, s);
     case 7: return go<2, 1>(a, s);
     case 8: return go<4, 1, 3>(a, s);
+    case 9: return go<4, 2, 3, 4, 1>(a, s);
+    case 10: return go<4, 1, 3, 4, 1, 4>(a, s);
+    case 11: return go<4, 2, 3, 4, 1, 3>(a, s);
+    case 12: return go<8, 2, 3, 4, 1>(a, s);
+    case 13: return go<6, 2, 3, 4, 1>(a, s);
+    case 14: return go<8, 1, 3, 4, 1>(a, s);
     default: return set_error("unknown fused FP32 launch configuration %d", cfg);
   }
 }

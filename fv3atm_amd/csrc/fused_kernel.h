@@ -349,23 +349,27 @@ __device__ __forceinline__ void ring_issue(const FusedArgs<float> &a, const Stri
   float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
   const long long frow = (long long)(x.fc - a.jsd) * nid;
   const long long gxrow = (long long)(x.rc - a.jsd) * (nid + 1) + (a.is - a.isd);
+  // uniform row base (scalar) + the lane's column offset in that array's
+  // row layout (cell, x-face, y-face column; dp1 as two 32-bit halves)
   for (int j = wave; j < LR_NL; j += W) {
-    const float *p;
+    const float *base;
+    int kind;
     switch (j) {
-      case 0: p = a.area + sc.tg + x.orow + sc.lo_q; break;
-      case 1: p = sc.cyk + (long long)(x.fc - a.js) * nid + sc.lo_q; break;
-      case 2: p = sc.cxk + (long long)(x.rc - a.jsd) * sc.nix + sc.lo_fx; break;
-      case 3: p = sc.mfyk + (long long)(x.fc - a.js) * sc.niy + sc.lo_mx; break;
-      case 4: p = sc.mfxk + (long long)(x.mc - a.js) * sc.nix + sc.lo_fx; break;
-      case 5: p = a.rarea + sc.tg + x.corow + sc.lo_q; break;
-      case 6: p = a.gyp + sc.tgf + frow + sc.lo_q; break;
-      case 7: p = a.gym + sc.tgf + frow + sc.lo_q; break;
-      case 8: p = a.gxp + sc.tgf + gxrow + sc.lo_fx; break;
-      case 9: p = a.gxm + sc.tgf + gxrow + sc.lo_fx; break;
-      case 10: p = (const float *)(sc.dp1k + x.corow + sc.lo_q); break;
-      default: p = (const float *)(sc.dp1k + x.corow + sc.lo_q) + 1; break;
+      case 0: base = a.area + sc.tg + x.orow; kind = 0; break;
+      case 1: base = sc.cyk + (long long)(x.fc - a.js) * nid; kind = 0; break;
+      case 2: base = sc.cxk + (long long)(x.rc - a.jsd) * sc.nix; kind = 1; break;
+      case 3: base = sc.mfyk + (long long)(x.fc - a.js) * sc.niy; kind = 2; break;
+      case 4: base = sc.mfxk + (long long)(x.mc - a.js) * sc.nix; kind = 1; break;
+      case 5: base = a.rarea + sc.tg + x.corow; kind = 0; break;
+      case 6: base = a.gyp + sc.tgf + frow; kind = 0; break;
+      case 7: base = a.gym + sc.tgf + frow; kind = 0; break;
+      case 8: base = a.gxp + sc.tgf + gxrow; kind = 1; break;
+      case 9: base = a.gxm + sc.tgf + gxrow; kind = 1; break;
+      case 10: base = (const float *)(sc.dp1k + x.corow); kind = 3; break;
+      default: base = (const float *)(sc.dp1k + x.corow) + 1; kind = 3; break;
     }
-    lds_dma(p, slot[j]);
+    const int off = kind == 0 ? sc.lo_q : (kind == 1 ? sc.lo_fx : (kind == 2 ? sc.lo_mx : 2 * sc.lo_q));
+    lds_dma(base + off, slot[j]);
   }
   // inactive tracers load tracer 0: every wave issues the same count
 #pragma unroll
@@ -424,7 +428,9 @@ __device__ __forceinline__ void ring_wait(int ahead) {
 // OCC: minimum workgroups per CU asked of the register allocator.
 template <typename T, int W, int NT, int HO, int EK, int OCC>
 __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T> a) {
-  const int lane = threadIdx.x & (FT_LANES - 1), wave = threadIdx.x / FT_LANES;
+  const int lane = threadIdx.x & (FT_LANES - 1);
+  // wave index as a scalar: everything derived from it stays in SGPRs
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / FT_LANES);
   const int k = blockIdx.y;
   const int nspl = a.nsplit[k];
   // whole block: this level is done, or needs more passes than are launched

@@ -128,9 +128,10 @@ __device__ __forceinline__ T al2(T qm, T q0, T dmm, T dm0) {
   return T(0.5) * (qm + q0) + K<T>::r3 * (dmm - dm0);
 }
 // cube-edge face value between b and c from cells a b c e with spacings
+// da..de (x / y: exact division for FP64, reciprocal multiply for FP32)
 template <typename T>
 __device__ __forceinline__ T edge_face(T a, T b, T c, T e, T da, T db, T dc, T de) {
-  T xt = T(0.5) * (((T(2) * db + da) * b - db * a) / (da + db) + ((T(2) * dc + de) * c - dc * e) / (dc + de));
+  T xt = T(0.5) * (divide((T(2) * db + da) * b - db * a, da + db) + divide((T(2) * dc + de) * c - dc * e, dc + de));
   xt = fmx(xt, fmn(fmn3(a, b, c), e));
   xt = fmn(xt, fmx(fmx3(a, b, c), e));
   return xt;
@@ -171,39 +172,31 @@ __device__ __forceinline__ int edge_role(int m, int is, int ie, int npx, bool ed
   return 0;
 }
 
-// (bl, br) of a cell with a cube-edge role (xppm/yppm edge code).
-// d0..d4: spacings of cells m-2..m+2.
+// (bl, br) of a cell with a cube-edge role (xppm/yppm edge code): role
+// 1..3 are cells 0, 1, 2 of the line, 4..6 cells npx-2, npx-1, npx.  Every
+// candidate is computed and the role selects (no divergent branches across
+// the lanes of an x-sweep); role 0 keeps the interior (bl, br).  efl / efr:
+// the edge face values on the cell's left / right.
 template <typename T>
-__device__ __forceinline__ void edge_coef(T qm2, T qm1, T q0, T qp1, T qp2, T dmm1, T dm0, T dmp1,
-                                       T al0, T alp1, int role, T d0, T d1, T d2, T d3, T d4,
-                                       T &bl, T &br) {
-  switch (role) {
-    case 1:
-      bl = K<T>::s14 * dmm1 + K<T>::s11 * (qm1 - q0);
-      br = edge_face(qm1, q0, qp1, qp2, d1, d2, d3, d4) - q0;
-      break;
-    case 2:
-      bl = edge_face(qm2, qm1, q0, qp1, d0, d1, d2, d3) - q0;
-      br = K<T>::s15 * q0 + K<T>::s11 * qp1 - K<T>::s14 * dmp1 - q0;
-      break;
-    case 3:
-      bl = K<T>::s15 * qm1 + K<T>::s11 * q0 - K<T>::s14 * dm0 - q0;
-      br = alp1 - q0;
-      break;
-    case 4:
-      bl = al0 - q0;
-      br = K<T>::s15 * qp1 + K<T>::s11 * q0 + K<T>::s14 * dm0 - q0;
-      break;
-    case 5:
-      bl = K<T>::s15 * q0 + K<T>::s11 * qm1 + K<T>::s14 * dmm1 - q0;
-      br = edge_face(qm1, q0, qp1, qp2, d1, d2, d3, d4) - q0;
-      break;
-    default:
-      bl = edge_face(qm2, qm1, q0, qp1, d0, d1, d2, d3) - q0;
-      br = K<T>::s11 * (qp1 - q0) - K<T>::s14 * dmp1;
-      break;
+__device__ __forceinline__ void edge_coef_sel(T qm1, T q0, T qp1, T dmm1, T dm0, T dmp1, T al0, T alp1,
+                                              int role, T efl, T efr, T &bl, T &br) {
+  const T b1 = K<T>::s14 * dmm1 + K<T>::s11 * (qm1 - q0);
+  const T b3 = K<T>::s15 * qm1 + K<T>::s11 * q0 - K<T>::s14 * dm0 - q0;
+  const T b4 = al0 - q0;
+  const T b5 = K<T>::s15 * q0 + K<T>::s11 * qm1 + K<T>::s14 * dmm1 - q0;
+  const T bL = efl - q0;  // roles 2, 6
+  const T r2 = K<T>::s15 * q0 + K<T>::s11 * qp1 - K<T>::s14 * dmp1 - q0;
+  const T r3 = alp1 - q0;
+  const T r4 = K<T>::s15 * qp1 + K<T>::s11 * q0 + K<T>::s14 * dm0 - q0;
+  const T r6 = K<T>::s11 * (qp1 - q0) - K<T>::s14 * dmp1;
+  const T rR = efr - q0;  // roles 1, 5
+  T l = role == 1 ? b1 : (role == 3 ? b3 : (role == 4 ? b4 : (role == 5 ? b5 : bL)));
+  T r = role == 2 ? r2 : (role == 3 ? r3 : (role == 4 ? r4 : (role == 6 ? r6 : rR)));
+  pert_ppm1(q0, l, r, 1);
+  if (role != 0) {
+    bl = l;
+    br = r;
   }
-  pert_ppm1(q0, bl, br, 1);
 }
 
 // (bl, br) of an interior cell for the monotone orders
@@ -267,10 +260,11 @@ __device__ __forceinline__ T xsweep(T q, T c, int i, int is, int ie, int npx, bo
   T bl, br;
   cell_coef(qm2, qm1, q, qp1, qp2, dmm1, dm, dmp1, al, alp1, iord, bl, br);
   if (edge_strip) {
+    // the left edge face of this cell is the right one of its left neighbour
     const int role = edge_role(i, is, ie, npx, edge);
-    const T d1 = from_left(dxl), d3 = from_right(dxl);
-    const T d0 = from_left(d1), d4 = from_right(d3);
-    if (role != 0) edge_coef(qm2, qm1, q, qp1, qp2, dmm1, dm, dmp1, al, alp1, role, d0, d1, dxl, d3, d4, bl, br);
+    const T d1 = from_left(dxl), d3 = from_right(dxl), d4 = from_right(d3);
+    const T efr = edge_face(qm1, q, qp1, qp2, d1, dxl, d3, d4);
+    edge_coef_sel(qm1, q, qp1, dmm1, dm, dmp1, al, alp1, role, from_left(efr), efr, bl, br);
   }
   const T blm = from_left(bl), brm = from_left(br);
   return face_flux2(c, qm1, blm, brm, q, bl, br);
@@ -299,9 +293,12 @@ __device__ __forceinline__ T ypipe_step(YPipe<T> &p, T qnew, T c, int iord, int 
   p.al[s1] = al2(p.w[s2], p.w[s1], p.dm[s2], p.dm[s1]);
   cell_coef(p.w[s4], p.w[s3], p.w[s2], p.w[s1], p.w[s0], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2],
             p.al[s1], iord, p.bl[s2], p.br[s2]);
-  if (role != 0)
-    edge_coef(p.w[s4], p.w[s3], p.w[s2], p.w[s1], p.w[s0], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2],
-              p.al[s1], role, d0, d1, d2, d3, d4, p.bl[s2], p.br[s2]);
+  if (role != 0) {  // uniform over the wave
+    const T efl = edge_face(p.w[s4], p.w[s3], p.w[s2], p.w[s1], d0, d1, d2, d3);
+    const T efr = edge_face(p.w[s3], p.w[s2], p.w[s1], p.w[s0], d1, d2, d3, d4);
+    edge_coef_sel(p.w[s3], p.w[s2], p.w[s1], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2], p.al[s1], role, efl,
+                  efr, p.bl[s2], p.br[s2]);
+  }
   return face_flux2(c, p.w[s3], p.bl[s3], p.br[s3], p.w[s2], p.bl[s2], p.br[s2]);
 }
 
