@@ -142,22 +142,23 @@ def profiled_traffic(n, npz, nq, cfg, rows):
     return None, None
 
 
-def cpu_baseline(case):
-    """FP64 oracle on the full tile, every level, one tracer (+ pt remap)."""
+def cpu_baseline(case, nq=2):
+    """FP64 oracle on the full tile, every level, nq tracers (+ pt remap)."""
     from oracle import oracle
     import copy
     c1 = copy.copy(case)
-    c1.nq = 1
-    c1.q = np.asfortranarray(case.meta["q_bases"][0][0][:, :, None, None] *
-                             case.meta["lev"][None, None, :, None])
+    c1.nq = nq
+    bases = case.meta["q_bases"][0]
+    c1.q = np.asfortranarray(np.stack([bases[i % len(bases)][:, :, None] * case.meta["lev"][None, None, :]
+                                       for i in range(nq)], axis=-1))
     t0 = time.perf_counter()
     adv = oracle.tracer_2d(c1, "1L")
     oracle.remap_tracers(c1, adv["q"], c1.dp1, c1.pt)
     t = time.perf_counter() - t0
-    cells = case.n * case.n * case.npz * 1
+    cells = case.n * case.n * case.npz * nq
     return dict(value=cells / t / 1e9, unit="Gcell-tracer updates/s", cores=1, kind="port",
                 sample=f"FP64 oracle (C restatement), full C{case.n} tile, {case.npz} levels, "
-                       f"1 tracer (+pt remap), one step in {t:.1f} s on one host core")
+                       f"{nq} tracers (+pt remap), one step in {t:.1f} s on one host core")
 
 
 def main():
@@ -173,9 +174,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="first run a 1 GiB copy kernel (reference for HBM byte counters)")
-    ap.add_argument("--launch", type=int, default=-1,
-                    help="fused launch configuration; -1 tunes over all of them first")
-    ap.add_argument("--rows", type=int, default=0, help="rows per block (0: default or tuned)")
+    ap.add_argument("--launch", type=int, default=0,
+                    help="fused launch configuration (0: the library default); -1 tunes over all "
+                         "of them and the rows per block first")
+    ap.add_argument("--rows", type=int, default=0, help="rows per block (0: the library default, 60)")
     ap.add_argument("--graph", action="store_true",
                     help="time the step as hipGraph replays (graph-safe mode)")
     args = ap.parse_args()
@@ -285,7 +287,7 @@ def main():
         launch, rows = best[1], best[2]
     fa.set_launch(ctx, launch, rows)
     out["launch_cfg"] = launch
-    out["rows_per_block"] = rows or 90
+    out["rows_per_block"] = rows or 60
     ms, t_adv, t_rem = run_form(fa, ctx, fa.FORM_FUSED32, d, case, args.steps, args.warmup, cap,
                                 torch, timing=True)
     log(f"fused FP32: {ms:.2f} ms/step (transport {np.mean(t_adv):.2f} ms, remap {np.mean(t_rem):.2f} ms)")
@@ -305,7 +307,7 @@ def main():
     ta = float(np.mean(t_adv)) * 1e-3
     # one launch = one sub-cycle pass over every level (the interior and frame
     # grids of k_fused_tp back to back); cap passes per step
-    traffic, tsrc = profiled_traffic(N, L, Q, launch, rows or 90)
+    traffic, tsrc = profiled_traffic(N, L, Q, launch, rows or 60)
     roof = dict(bound="hbm", achieved=round(b_adv / ta / 1e9, 1), peak=HBM_PEAK / 1e9,
                 unit="GB/s", frac=round(b_adv / ta / HBM_PEAK, 4), traffic=traffic)
     out["algorithmic_bytes_per_launch"] = b_adv // cap

@@ -108,7 +108,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   a.tmx = d.smx * npz;
   a.tmy = d.smy * npz;
   a.tdp = d.s2 * npz;
-  a.H = std::min(nj, c.fused_h > 0 ? c.fused_h : 90);
+  a.H = std::min(nj, c.fused_h > 0 ? c.fused_h : 60);
   a.nstrip = (ni + FT_BX - 1) / FT_BX;
   a.nvblk = (nj + a.H - 1) / a.H;
   // interior blocks: every lane's column i0-3..i0+60 and every input row

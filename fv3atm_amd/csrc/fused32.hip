@@ -32,9 +32,9 @@ static int go(const FusedArgs<float> &a, hipStream_t s) {
 
 int launch_fused_f32(const FusedArgs<float> &a, int cfg, hipStream_t s) {
   switch (cfg) {
-    case 0: return go<4, 1>(a, s);
+    case 0: return go<4, 2>(a, s);
     case 1: return go<8, 1>(a, s);
-    case 2: return go<4, 2>(a, s);
+    case 2: return go<4, 1>(a, s);
     case 3: return go<2, 2>(a, s);
     case 4: return go<6, 1>(a, s);
     case 5: return go<1, 2>(a, s);

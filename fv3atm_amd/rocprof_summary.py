@@ -2,6 +2,11 @@
 
   python -m fv3atm_amd.rocprof_summary stats DIR
       per-kernel calls, total and mean duration from the kernel trace
+  python -m fv3atm_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR --kernel K
+          --launches L --calib-bytes B [--tag "k=v ..."]
+      HBM bytes of every dispatch whose name contains K (FETCH_SIZE run plus
+      WRITE_SIZE run, each calibrated as below), summed and divided by L
+      launches; prints a 'bench-traffic:' line carrying the tag
   python -m fv3atm_amd.rocprof_summary counters DIR [--calib-bytes B]
       per-kernel mean of every collected counter per dispatch; with
       --calib-bytes, the counter of the calibration kernel (the one whose
@@ -87,10 +92,32 @@ def counter_means(d):
     return agg
 
 
+def calibrated_sum(d, counter, kernel, calib_name, calib_bytes):
+    """(sum over dispatches of kernels named like `kernel`, dispatch count,
+    calibration factor) of one counter, in bytes."""
+    agg = counter_means(d)
+    factor = None
+    for (n, cn), (c, s) in agg.items():
+        if cn == counter and calib_name.lower() in n.lower():
+            factor = calib_bytes / (s / c * 1024.0)
+    if factor is None:
+        raise SystemExit(f"no calibration kernel '{calib_name}' with {counter} under {d}")
+    tot, cnt = 0.0, 0
+    for (n, cn), (c, s) in agg.items():
+        if cn == counter and kernel in n:
+            tot += s * 1024.0 * factor
+            cnt += c
+    return tot, cnt, factor
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "counters"])
+    ap.add_argument("mode", choices=["stats", "counters", "traffic"])
     ap.add_argument("dir")
+    ap.add_argument("dir2", nargs="?")
+    ap.add_argument("--kernel", default="k_fused_tp")
+    ap.add_argument("--launches", type=int, default=1)
+    ap.add_argument("--tag", default="")
     ap.add_argument("--calib-bytes", type=float, default=0.0)
     ap.add_argument("--calib-name", default="bitwise_not")
     ap.add_argument("--top", type=int, default=12)
@@ -102,6 +129,15 @@ def main(argv=None):
         w("| kernel | calls | total ms | mean us | share |\n|---|---:|---:|---:|---:|\n")
         for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[: a.top]:
             w(f"| `{clip(n)}` | {c} | {t / 1e6:.3f} | {t / c / 1e3:.1f} | {100 * t / tot:.1f}% |\n")
+        return
+    if a.mode == "traffic":
+        f, nf, ff = calibrated_sum(a.dir, "FETCH_SIZE", a.kernel, a.calib_name, a.calib_bytes)
+        wr, nw, fw = calibrated_sum(a.dir2, "WRITE_SIZE", a.kernel, a.calib_name, a.calib_bytes)
+        per = (f + wr) / a.launches
+        w(f"kernels matching `{a.kernel}`: {nf} dispatches (FETCH run), {nw} (WRITE run)\n")
+        w(f"FETCH_SIZE factor {ff:.3f}: {f:.4e} B; WRITE_SIZE factor {fw:.3f}: {wr:.4e} B\n")
+        w(f"HBM bytes per launch ({a.launches} launches): {per:.4e}\n")
+        w(f"bench-traffic: {a.tag} bytes_per_pass={per:.0f}\n")
         return
     agg = counter_means(a.dir)
     factor = {}

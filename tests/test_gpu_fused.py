@@ -23,11 +23,11 @@ def _dev(a, dtype):
         device="cuda", dtype=dtype)
 
 
-def run_step(case, form, nsplit_cap=0, rows=0):
+def run_step(case, form, nsplit_cap=0, rows=0, cfg=0):
     dt = torch.float32 if form == fa.FORM_FUSED32 else torch.float64
     ctx = fa.Context(case.grids, case.npz)
-    if rows:
-        fa.set_launch(ctx, 0, rows, 0)
+    if rows or cfg:
+        fa.set_launch(ctx, cfg, rows, 0)
     q_in = _dev(case.q, dt)
     q_out = torch.zeros_like(q_in)
     pt = _dev(case.pt, dt) if case.pt is not None else None
@@ -146,3 +146,22 @@ def test_step_six_tiles_interior_blocks(gpu_ready):
         e = rel_l2(interior(got32["q"][..., iq, :]), interior(ref["q"][..., iq, :]))
         print(f"FP32 tracer {iq}: rel L2 {e:.2e}")
         assert e <= 1e-5
+
+
+def test_step_every_launch_shape(gpu_ready):
+    # every launch shape (waves per workgroup x tracers per wave, lean and
+    # edge kinds, occupancy) on a case with all three block kinds and a
+    # partial last round of tracers
+    case = cases.make_case("C192L4Q7jet", n=192, npz=4, nq=7, jet=True, courant=2.1, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=23, with_pt=True)
+    ref = reference(case)
+    base = None
+    for cfg in range(fa.N_LAUNCH_CFG):
+        got = run_step(case, fa.FORM_FUSED32, rows=30, cfg=cfg)
+        errs = [rel_l2(interior(got["q"][:, :, :, iq]), interior(ref["q"][:, :, :, iq])) for iq in range(case.nq)]
+        print(f"cfg {cfg}: max rel L2 vs oracle {max(errs):.2e}", flush=True)
+        assert max(errs) <= 1e-5
+        if base is None:
+            base = got["q"]
+        else:
+            assert rel_l2(interior(got["q"]), interior(base)) <= 1e-6

@@ -43,3 +43,20 @@ def test_stats_and_counters(tmp_path):
     # 512 KiB read for 1 MiB -> factor 2; k_a: 50 KiB mean -> 102400 bytes
     assert "factor 2.000" in text
     assert "1.0240e+05" in text
+
+
+def test_traffic_line_and_bench_lookup(tmp_path):
+    k1 = "void fv3::k_fused_tp<float, 4, 2, 8, 0, 1>(fv3::FusedArgs<float>)"
+    cal = "void at::vectorized_elementwise_kernel<4, bitwise_not_kernel_cuda>(int)"
+    for name, counter, vals in (("f", "FETCH_SIZE", (512, 10, 20)), ("w", "WRITE_SIZE", (1024, 3, 5))):
+        d = tmp_path / name
+        d.mkdir()
+        _write(d / "1_counter_collection.csv", ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"],
+               [[1, cal, counter, vals[0]], [2, k1, counter, vals[1]], [3, k1, counter, vals[2]]])
+    out = io.StringIO()
+    with contextlib.redirect_stdout(out):
+        rs.main(["traffic", str(tmp_path / "f"), str(tmp_path / "w"), "--kernel", "k_fused_tp",
+                 "--launches", "2", "--calib-bytes", str(1024 * 1024), "--tag", "n=8 npz=2 nq=1 cfg=0 rows=60"])
+    # fetch: (10 + 20) KiB x 2 = 61440 B; write: (3 + 5) KiB x 1 = 8192 B; per launch 34816 B
+    line = [l for l in out.getvalue().splitlines() if l.startswith("bench-traffic:")][0]
+    assert line == "bench-traffic: n=8 npz=2 nq=1 cfg=0 rows=60 bytes_per_pass=34816"
