@@ -34,12 +34,16 @@ def log(*a):
 
 
 def device_inputs(case, torch, dtype):
-    """Flat device tensors in the Fortran layouts the C-ABI takes."""
+    """Flat device tensors in the Fortran layouts the C-ABI takes (tiles outermost)."""
     f = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a.ravel(order="F"))).to("cuda", dt)
-    bases = torch.stack([torch.from_numpy(np.ascontiguousarray(b.T)) for b in case.meta["q_bases"][0]])
-    lev = torch.from_numpy(case.meta["lev"])
-    q = (bases.to("cuda", torch.float64)[:, None, :, :] * lev.to("cuda")[None, :, None, None]).to(dtype)
-    d = dict(q_in=q.reshape(-1).contiguous(), pt=f(case.pt, dtype), dp1=f(case.dp1, torch.float64),
+    lev = torch.from_numpy(case.meta["lev"]).to("cuda")
+    qs = []
+    for tb in case.meta["q_bases"]:  # q(i,j,k,iq,tile) = base(i,j) * lev(k)
+        bases = torch.stack([torch.from_numpy(np.ascontiguousarray(b.T)) for b in tb]).to("cuda", torch.float64)
+        qs.append((bases[:, None, :, :] * lev[None, :, None, None]).to(dtype))
+    q = torch.stack(qs) if len(qs) > 1 else qs[0]
+    pt = f(case.pt, dtype) if case.pt is not None else None
+    d = dict(q_in=q.reshape(-1).contiguous(), pt=pt, dp1=f(case.dp1, torch.float64),
              cx=f(case.cx, dtype), cy=f(case.cy, dtype), mfx=f(case.mfx, dtype),
              mfy=f(case.mfy, dtype), ak=f(case.ak, torch.float64), bk=f(case.bk, torch.float64))
     d["q_out"] = torch.zeros_like(d["q_in"])
@@ -161,6 +165,68 @@ def cpu_baseline(case, nq=2):
                        f"{nq} tracers (+pt remap), one step in {t:.1f} s on one host core")
 
 
+def run_sphere(args, torch, fa, cases):
+    """Six C384 faces, 127 levels, 20 tracers: the fused FP32 step against the
+    fused FP64 step (bit-identical to the oracle on small spheres), global
+    tracer mass over the sphere, and the FP32 step time.  Prints one JSON line."""
+    t0 = time.perf_counter()
+    case = cases.config(4, build_q=False)
+    N, L, Q, T = case.n, case.npz, case.nq, case.ntiles
+    log(f"case 6xC{N} L{L} Q{Q} built in {time.perf_counter() - t0:.1f}s")
+    nid = N + 6
+    area = torch.stack([torch.from_numpy(np.ascontiguousarray(g.area.T)) for g in case.grids]).cuda()
+    area = area[:, 3:-3, 3:-3]
+
+    def run(form, dtype):
+        ctx = fa.Context(case.grids, L)
+        d = device_inputs(case, torch, dtype)
+        fa.set_timing(ctx, True)
+        fa.step_dev(ctx, form, Q, d["q_in"], d["q_out"], None, d["dp1"], d["delp_out"], d["cx"], d["cy"],
+                    d["mfx"], d["mfy"], d["ak"], d["bk"], case.ptop, case.hord, case.kord_tr, 10, 184.0, 0)
+        torch.cuda.synchronize()
+        passes = fa.step_timing(ctx)[2]
+        return ctx, d, passes
+
+    def mass(q, dp, absolute=False):  # per tracer, interior cells of every face
+        q = q.view(T, Q, L, nid, nid)[:, :, :, 3:-3, 3:-3].double()
+        q = q.abs() if absolute else q
+        dp = dp.view(T, 1, L, nid, nid)[:, :, :, 3:-3, 3:-3]
+        return (q * dp * area[:, None, None]).sum(dim=(0, 2, 3, 4))
+
+    ctx64, d64, passes = run(fa.FORM_FUSED64, torch.float64)
+    m0 = mass(d64["q_in"], d64["dp1"])
+    mabs = mass(d64["q_in"], d64["dp1"], absolute=True)  # tracers of either sign
+    m64 = mass(d64["q_out"], d64["delp_out"])
+    q64 = d64["q_out"].view(T, Q, L, nid, nid)[:, :, :, 3:-3, 3:-3].clone()
+    ctx64.close()
+    del d64
+    torch.cuda.empty_cache()
+    ctx, d, _ = run(fa.FORM_FUSED32, torch.float32)
+    q32 = d["q_out"].view(T, Q, L, nid, nid)[:, :, :, 3:-3, 3:-3].double()
+    rel = (torch.linalg.vector_norm((q32 - q64).transpose(0, 1).reshape(Q, -1), dim=1) /
+           torch.linalg.vector_norm(q64.transpose(0, 1).reshape(Q, -1), dim=1)).max().item()
+    m32 = mass(d["q_out"], d["delp_out"])
+    dm64 = ((m64 - m0).abs() / mabs).max().item()
+    dm32 = ((m32 - m0).abs() / mabs).max().item()
+    del q32, q64
+    log(f"{passes} passes; FP32 vs FP64 rel L2 max {rel:.2e}; global mass drift FP64 {dm64:.2e}, FP32 {dm32:.2e}")
+    ms, t_adv, t_rem = run_form(fa, ctx, fa.FORM_FUSED32, d, case, args.steps, args.warmup, passes, torch,
+                                timing=True)
+    value = T * N * N * L * Q / (ms * 1e-3) / 1e9
+    log(f"fused FP32 sphere: {ms:.2f} ms/step (transport {np.mean(t_adv):.2f}, remap {np.mean(t_rem):.2f})")
+    print(json.dumps({
+        "metric": f"Gcell-tracer updates/s 6xC{N}L{L}x{Q} tracers, fused FP32 step", "value": round(value, 3),
+        "unit": "Gcell-tracer updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"6 x C{N} L{L} x {Q} tracers, solid-body rotation over the cube corners: "
+                               f"tracer_2d_1L hord 8 + kord 9 tracer remap, fused FP32 form",
+                   "n": N, "npz": L, "nq": Q, "ntiles": T, "passes": passes},
+        "transport_ms": round(float(np.mean(t_adv)), 3), "remap_ms": round(float(np.mean(t_rem)), 3),
+        "fp32_rel_l2_max": rel, "mass_drift_fp64": dm64, "mass_drift_fp32": dm32}), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,10 +246,12 @@ def main():
     ap.add_argument("--rows", type=int, default=0, help="rows per block (0: the library default, 60)")
     ap.add_argument("--graph", action="store_true",
                     help="time the step as hipGraph replays (graph-safe mode)")
+    ap.add_argument("--sphere", action="store_true",
+                    help="the six-face C384 L127 Q20 sphere instead of the C768 tile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "the multi-GPU layout (one cube tile per GPU) is described in DESIGN.md",
+              "across GPUs the path would shard one cube tile per GPU, which is not built",
               file=sys.stderr)
         sys.exit(2)
 
@@ -192,6 +260,9 @@ def main():
     from fv3atm_amd import cases
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.sphere:
+        run_sphere(args, torch, fa, cases)
+        return
     if args.calib:  # reads 1 GiB and writes 1 GiB in one elementwise kernel
         x = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
         y = torch.empty_like(x)
