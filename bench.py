@@ -244,6 +244,7 @@ def main():
                     help="fused launch configuration (0: the library default); -1 tunes over all "
                          "of them and the rows per block first")
     ap.add_argument("--rows", type=int, default=0, help="rows per block (0: the library default, 60)")
+    ap.add_argument("--remap", type=int, default=0, help="remap kernel (0: the library default)")
     ap.add_argument("--graph", action="store_true",
                     help="time the step as hipGraph replays (graph-safe mode)")
     ap.add_argument("--sphere", action="store_true",
@@ -340,7 +341,7 @@ def main():
     if launch < 0:  # untimed tuning of the launch shape, then rows per block
         best = None
         for cfg in range(fa.N_LAUNCH_CFG):
-            fa.set_launch(ctx, cfg, rows)
+            fa.set_launch(ctx, cfg, rows, args.remap)
             _, ta_c, _ = run_form(fa, ctx, fa.FORM_FUSED32, d, case, 2, 1, cap, torch, timing=True)
             t = float(np.min(ta_c))
             log(f"launch cfg {cfg} rows {rows or 'default'}: transport {t:.2f} ms")
@@ -349,15 +350,16 @@ def main():
         for r in (42, 48, 60, 72, 90):
             if r == rows or r > N:
                 continue
-            fa.set_launch(ctx, best[1], r)
+            fa.set_launch(ctx, best[1], r, args.remap)
             _, ta_c, _ = run_form(fa, ctx, fa.FORM_FUSED32, d, case, 2, 1, cap, torch, timing=True)
             t = float(np.min(ta_c))
             log(f"launch cfg {best[1]} rows {r}: transport {t:.2f} ms")
             if t < best[0]:
                 best = (t, best[1], r)
         launch, rows = best[1], best[2]
-    fa.set_launch(ctx, launch, rows)
+    fa.set_launch(ctx, launch, rows, args.remap)
     out["launch_cfg"] = launch
+    out["remap_kernel"] = args.remap
     out["rows_per_block"] = rows or 60
     ms, t_adv, t_rem = run_form(fa, ctx, fa.FORM_FUSED32, d, case, args.steps, args.warmup, cap,
                                 torch, timing=True)
