@@ -245,6 +245,8 @@ def main():
                          "of them and the rows per block first")
     ap.add_argument("--rows", type=int, default=0, help="rows per block (0: the library default, 60)")
     ap.add_argument("--remap", type=int, default=0, help="remap kernel (0: the library default)")
+    ap.add_argument("--serial-kinds", action="store_true",
+                    help="run the transport's edge kernels after the lean kernel on one stream")
     ap.add_argument("--graph", action="store_true",
                     help="time the step as hipGraph replays (graph-safe mode)")
     ap.add_argument("--sphere", action="store_true",
@@ -358,7 +360,9 @@ def main():
                 best = (t, best[1], r)
         launch, rows = best[1], best[2]
     fa.set_launch(ctx, launch, rows, args.remap)
+    fa.set_concurrency(ctx, not args.serial_kinds)
     out["launch_cfg"] = launch
+    out["concurrent_kinds"] = not args.serial_kinds
     out["remap_kernel"] = args.remap
     out["rows_per_block"] = rows or 60
     ms, t_adv, t_rem = run_form(fa, ctx, fa.FORM_FUSED32, d, case, args.steps, args.warmup, cap,

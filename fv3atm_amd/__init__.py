@@ -204,6 +204,12 @@ def step_timing(ctx: Context):
     return a.value, r.value, p.value
 
 
+def set_concurrency(ctx: Context, on: bool = True):
+    """Edge kernels of the fused transport beside the lean kernel on two
+    auxiliary streams (True, the default) or after it on the step's stream."""
+    check(_ffi.lib().fv3_ctx_set_concurrency(ctx.handle, 1 if on else 0), "fv3_ctx_set_concurrency")
+
+
 def set_launch(ctx: Context, cfg: int = 0, rows: int = 0, remap: int = 0):
     """Kernel choices of the FP32 step: transport launch shape (cfg, rows;
     results unchanged) and remap kernel (0 column-shared, 1 per column)."""

@@ -180,6 +180,17 @@ int fv3_ctx_create(const fv3_bounds *bd, int npx, int npy, int npz, int ntiles, 
     delete h;
     return set_error("hipStreamCreate failed");
   }
+  for (auto &a : c.aux)
+    if (hipStreamCreateWithFlags(&a, hipStreamNonBlocking) != hipSuccess) {
+      fv3_ctx_destroy(h);
+      return set_error("hipStreamCreateWithFlags failed");
+    }
+  if (hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.join_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.join_ev[1], hipEventDisableTiming) != hipSuccess) {
+    fv3_ctx_destroy(h);
+    return set_error("hipEventCreateWithFlags failed");
+  }
   const long long s2 = c.d.s2, sdx = (long long)c.d.nid * (c.d.njd + 1), sdy = (long long)(c.d.nid + 1) * c.d.njd;
   const long long per_tile = s2 * 4 + sdx * 2 + sdy * 2 + s2 * 9;
   if (hipMalloc(&c.g64, sizeof(double) * per_tile * ntiles) != hipSuccess ||
@@ -242,6 +253,14 @@ int fv3_ctx_destroy(fv3_ctx *h) {
   (void)hipFree(c.stage);
   (void)hipFree(c.step_flag);
   for (auto &e : c.ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &a : c.aux)
+    if (a) {
+      (void)hipStreamSynchronize(a);
+      (void)hipStreamDestroy(a);
+    }
+  if (c.fork_ev) (void)hipEventDestroy(c.fork_ev);
+  for (auto &e : c.join_ev)
     if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
@@ -682,6 +701,14 @@ int fv3_step_timing(fv3_ctx *h, double *transport_ms, double *remap_ms, int *pas
   *transport_ms = a;
   *remap_ms = b;
   *passes = c.last_passes;
+  return 0;
+}
+
+int fv3_ctx_set_concurrency(fv3_ctx *h, int on) {
+  if (!h) return set_error("NULL context");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.concurrent_kinds = on != 0;
   return 0;
 }
 

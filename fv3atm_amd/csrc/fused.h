@@ -9,8 +9,12 @@ namespace fv3 {
 // launch configurations (waves per workgroup x tracers per wave); cfg 0 is
 // the default, the others are the autotuner's candidates
 constexpr int FUSED32_NCFG = 15;
-int launch_fused_f64(const FusedArgs<double> &a, hipStream_t s);
-int launch_fused_f32(const FusedArgs<float> &a, int cfg, hipStream_t s);
+// streams of the three kernel kinds (lean, edge rows, edge strips)
+struct KindStreams {
+  hipStream_t k[3];
+};
+int launch_fused_f64(const FusedArgs<double> &a, const KindStreams &s);
+int launch_fused_f32(const FusedArgs<float> &a, int cfg, const KindStreams &s);
 
 // per-level sub-cycle counts on the device (cmax, nsplit; no scaling)
 template <typename T>

@@ -11,17 +11,17 @@
 namespace fv3 {
 
 template <int HO>
-static void go64(const FusedArgs<double> &a, hipStream_t s) {
+static void go64(const FusedArgs<double> &a, const KindStreams &s) {
   const dim3 b(FT_LANES * 4);
   int n0, n1, n2;
   fused_grid_split(a, n0, n1, n2);
   auto grid = [&](int n) { return dim3((unsigned)n, (unsigned)a.npz, (unsigned)a.ntiles); };
-  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 0, 1>), grid(n0), b, 0, s, a);
-  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 1, 1>), grid(n1), b, 0, s, a);
-  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 2, 1>), grid(n2), b, 0, s, a);
+  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 0, 1>), grid(n0), b, 0, s.k[0], a);
+  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 1, 1>), grid(n1), b, 0, s.k[1], a);
+  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 2, 1>), grid(n2), b, 0, s.k[2], a);
 }
 
-int launch_fused_f64(const FusedArgs<double> &a, hipStream_t s) {
+int launch_fused_f64(const FusedArgs<double> &a, const KindStreams &s) {
   if (a.hord == 8)
     go64<8>(a, s);
   else
@@ -154,14 +154,29 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   }
   const bool timing = c.timing && !c.graph_safe;
   if (timing) FV3_HIP(hipEventRecord(c.ev[0], s));
+  // each pass: the edge kinds fork onto the auxiliary streams and join back
+  // before the next pass (which reads every face's previous pass)
+  const bool conc = c.concurrent_kinds && c.aux[0] && c.aux[1];
+  const KindStreams ks{{s, conc ? c.aux[0] : s, conc ? c.aux[1] : s}};
   for (int it = 1; it <= cap; it++) {
     a.it = it;
+    if (conc) {
+      FV3_HIP(hipEventRecord(c.fork_ev, s));
+      FV3_HIP(hipStreamWaitEvent(c.aux[0], c.fork_ev, 0));
+      FV3_HIP(hipStreamWaitEvent(c.aux[1], c.fork_ev, 0));
+    }
     int rc;
     if constexpr (std::is_same<T, double>::value)
-      rc = launch_fused_f64(a, s);
+      rc = launch_fused_f64(a, ks);
     else
-      rc = launch_fused_f32(a, c.fused_cfg, s);
+      rc = launch_fused_f32(a, c.fused_cfg, ks);
     if (rc) return rc;
+    if (conc) {
+      FV3_HIP(hipEventRecord(c.join_ev[0], c.aux[0]));
+      FV3_HIP(hipEventRecord(c.join_ev[1], c.aux[1]));
+      FV3_HIP(hipStreamWaitEvent(s, c.join_ev[0], 0));
+      FV3_HIP(hipStreamWaitEvent(s, c.join_ev[1], 0));
+    }
   }
   if (nsplit_cap > 0) {
     hipLaunchKernelGGL(k_nsplit_check, dim3(1), dim3(1), 0, s, nspl, npz, cap, flag);

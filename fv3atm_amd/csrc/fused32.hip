@@ -10,18 +10,18 @@ namespace fv3 {
 // occupancy asked of the edge kinds' register allocation (the lean kind with
 // one tracer per wave fits three waves per SIMD without spilling)
 template <int W, int NT, int HO, int OCC_E, int WE, int NTE, int OCC_L0>
-static void go2(const FusedArgs<float> &a, hipStream_t s) {
+static void go2(const FusedArgs<float> &a, const KindStreams &s) {
   constexpr int OCC_L = OCC_L0 > 0 ? OCC_L0 : (NT > 1 ? 1 : (W >= 4 ? 12 / W : 3));
   int n0, n1, n2;
   fused_grid_split(a, n0, n1, n2);
   auto grid = [&](int n) { return dim3((unsigned)n, (unsigned)a.npz, (unsigned)a.ntiles); };
-  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 0, OCC_L>), grid(n0), dim3(FT_LANES * W), 0, s, a);
-  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 1, OCC_E>), grid(n1), dim3(FT_LANES * WE), 0, s, a);
-  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 2, OCC_E>), grid(n2), dim3(FT_LANES * WE), 0, s, a);
+  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 0, OCC_L>), grid(n0), dim3(FT_LANES * W), 0, s.k[0], a);
+  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 1, OCC_E>), grid(n1), dim3(FT_LANES * WE), 0, s.k[1], a);
+  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 2, OCC_E>), grid(n2), dim3(FT_LANES * WE), 0, s.k[2], a);
 }
 
 template <int W, int NT, int OCC_E = 1, int WE = W, int NTE = NT, int OCC_L = 0>
-static int go(const FusedArgs<float> &a, hipStream_t s) {
+static int go(const FusedArgs<float> &a, const KindStreams &s) {
   if (a.hord == 8)  // the production order, specialised at compile time
     go2<W, NT, 8, OCC_E, WE, NTE, OCC_L>(a, s);
   else
@@ -30,7 +30,7 @@ static int go(const FusedArgs<float> &a, hipStream_t s) {
   return 0;
 }
 
-int launch_fused_f32(const FusedArgs<float> &a, int cfg, hipStream_t s) {
+int launch_fused_f32(const FusedArgs<float> &a, int cfg, const KindStreams &s) {
   switch (cfg) {
     case 0: return go<4, 2>(a, s);
     case 1: return go<8, 1>(a, s);

@@ -142,6 +142,12 @@ int fv3_ctx_set_timing(fv3_ctx *ctx, int on);
  * 1 = one thread per column.  Transport results do not depend on cfg/rows;
  * the two remap kernels agree to FP32 rounding. */
 int fv3_ctx_set_launch(fv3_ctx *ctx, int cfg, int rows, int remap);
+/* No counterpart upstream: run the fused transport's two edge kernels (the
+ * blocks that reach a halo or cube-edge row or column) beside the lean
+ * interior kernel on two auxiliary streams of the context, forked from and
+ * joined to the step's stream every pass (on = 1, the default), or after it
+ * on the step's stream (on = 0).  Results are identical either way. */
+int fv3_ctx_set_concurrency(fv3_ctx *ctx, int on);
 int fv3_step_timing(fv3_ctx *ctx, double *transport_ms, double *remap_ms, int *passes);
 /* Graph-safe steps (no counterpart upstream): with on != 0, fv3_step_dev
  * makes no host synchronisation, allocation or memset once its scratch is

@@ -23,11 +23,12 @@ def _dev(a, dtype):
         device="cuda", dtype=dtype)
 
 
-def run_step(case, form, nsplit_cap=0, rows=0, cfg=0):
+def run_step(case, form, nsplit_cap=0, rows=0, cfg=0, concurrent=True):
     dt = torch.float32 if form == fa.FORM_FUSED32 else torch.float64
     ctx = fa.Context(case.grids, case.npz)
     if rows or cfg:
         fa.set_launch(ctx, cfg, rows, 0)
+    fa.set_concurrency(ctx, concurrent)
     q_in = _dev(case.q, dt)
     q_out = torch.zeros_like(q_in)
     pt = _dev(case.pt, dt) if case.pt is not None else None
@@ -165,3 +166,19 @@ def test_step_every_launch_shape(gpu_ready):
             base = got["q"]
         else:
             assert rel_l2(interior(got["q"]), interior(base)) <= 1e-6
+
+
+@pytest.mark.parametrize("form", [fa.FORM_FUSED32, fa.FORM_FUSED64])
+def test_step_concurrent_kinds_equal_serial(gpu_ready, form):
+    # the edge kernels on their own streams give the same bits as run after
+    # the lean kernel on the step's stream (one tile and six tiles)
+    for mk in (lambda: cases.make_case("C192L4Q3jet", n=192, npz=4, nq=3, jet=True, courant=2.1,
+                                       delp_kind=1, delp_amp=(0.05, 0.2), seed=29, with_pt=True),
+               lambda: cases.make_case("6xC128L4Q2", n=128, npz=4, nq=2, ntiles=6, courant=2.2,
+                                       delp_kind=1, delp_amp=(0.05, 0.2), seed=31, with_pt=True)):
+        case = mk()
+        a = run_step(case, form, rows=30, concurrent=True)
+        b = run_step(case, form, rows=30, concurrent=False)
+        print(case.name, "max |concurrent - serial|", float(np.abs(a["q"] - b["q"]).max()), flush=True)
+        assert np.array_equal(interior(a["q"]), interior(b["q"]))
+        assert np.array_equal(interior(a["pt"]), interior(b["pt"]))
