@@ -381,6 +381,66 @@ __device__ __forceinline__ void ring_issue(const FusedArgs<float> &a, const Stri
   }
 }
 
+// Lean blocks never clamp a row (every input row and every stencil row lies
+// inside the compute domain), so each level array's row address is affine in
+// the row: the wave's share of the level arrays is set up once per round as
+// (row-0 address, elements per row, lane offset) and each row costs one
+// multiply-add per array.  (The edge kinds, short of scalar registers, keep
+// the per-row address expressions.)
+template <int W>
+struct LeanRows {
+  static constexpr int M = (LR_NL + W - 1) / W;
+  const float *p[M];
+  int step[M], loff[M];
+};
+
+template <int W>
+__device__ __forceinline__ LeanRows<W> lean_rows(const FusedArgs<float> &a, const StripCtx<float> &sc,
+                                                 int wave) {
+  LeanRows<W> lr;
+  const int nid = a.nid;
+  const long long r0 = sc.j0 - NG;  // input row of t = 0
+#pragma unroll
+  for (int m = 0; m < LeanRows<W>::M; m++) {
+    const int j = wave + m * W;
+    const float *p = a.area;
+    int step = nid, loff = sc.lo_q;
+    switch (j) {
+      case 0: p = a.area + sc.tg + (r0 - a.jsd) * nid; break;
+      case 1: p = sc.cyk + (r0 - 2 - a.js) * nid; break;
+      case 2: p = sc.cxk + (r0 - a.jsd) * sc.nix; step = sc.nix; loff = sc.lo_fx; break;
+      case 3: p = sc.mfyk + (r0 - 2 - a.js) * sc.niy; step = sc.niy; loff = sc.lo_mx; break;
+      case 4: p = sc.mfxk + (r0 - 3 - a.js) * sc.nix; step = sc.nix; loff = sc.lo_fx; break;
+      case 5: p = a.rarea + sc.tg + (r0 - 3 - a.jsd) * nid; break;
+      case 6: p = a.gyp + sc.tgf + (r0 - 2 - a.jsd) * nid; break;
+      case 7: p = a.gym + sc.tgf + (r0 - 2 - a.jsd) * nid; break;
+      case 8: p = a.gxp + sc.tgf + (r0 - a.jsd) * (nid + 1) + (a.is - a.isd); step = nid + 1; loff = sc.lo_fx; break;
+      case 9: p = a.gxm + sc.tgf + (r0 - a.jsd) * (nid + 1) + (a.is - a.isd); step = nid + 1; loff = sc.lo_fx; break;
+      case 10: p = (const float *)(sc.dp1k + (r0 - 3 - a.jsd) * nid); step = 2 * nid; loff = 2 * sc.lo_q; break;
+      case 11: p = (const float *)(sc.dp1k + (r0 - 3 - a.jsd) * nid) + 1; step = 2 * nid; loff = 2 * sc.lo_q; break;
+      default: break;  // past the last array: not issued
+    }
+    lr.p[m] = p;
+    lr.step[m] = step;
+    lr.loff[m] = loff;
+  }
+  return lr;
+}
+
+template <int W, int NT>
+__device__ __forceinline__ void ring_issue_lean(const FusedArgs<float> &a, const StripCtx<float> &sc,
+                                                const TrState<float> (&st)[NT], const LeanRows<W> &lr,
+                                                LdsRing<W, NT, 1> &ring, int wave, int t) {
+  float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
+#pragma unroll
+  for (int m = 0; m < LeanRows<W>::M; m++)
+    if (wave + m * W < LR_NL) lds_dma(lr.p[m] + (long long)t * lr.step[m] + lr.loff[m], slot[wave + m * W]);
+  const long long orow = (long long)(sc.j0 - NG + t - a.jsd) * a.nid;
+#pragma unroll
+  for (int n = 0; n < NT; n++)  // inactive tracers load tracer 0: every wave issues the same count
+    lds_dma(st[n].qin + orow + sc.lo_q, slot[LR_NL + wave * NT + n]);
+}
+
 template <int W, int NT, int QN>
 __device__ __forceinline__ void ring_read(const LdsRing<W, NT, QN> &ring, int wave, int lane, int t,
                                           RowIn<float, NT> &in) {
@@ -516,13 +576,21 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
       // the previous round's last reads of every slot are done before the
       // prologue overwrites them
       if (rd > 0) ring_wait<W, NT, QN>(0);
-      for (int t = 0; t < LR_DEPTH - 1 && t < nrow; t++) ring_issue<W, NT, EK>(a, sc, st, ring, wave, t);
+      LeanRows<W> lr;
+      if constexpr (EK == 0) lr = lean_rows<W>(a, sc, wave);
+      auto issue = [&](int t) {
+        if constexpr (EK == 0)
+          ring_issue_lean<W, NT>(a, sc, st, lr, ring, wave, t);
+        else
+          ring_issue<W, NT, EK>(a, sc, st, ring, wave, t);
+      };
+      for (int t = 0; t < LR_DEPTH - 1 && t < nrow; t++) issue(t);
       for (int t0 = 0; t0 < nrow; t0 += 6) {
 #define FV3_RROW(U)                                                                  \
   if (t0 + U < nrow) {                                                               \
     const int t = t0 + U;                                                            \
     ring_wait<W, NT, QN>(min(LR_DEPTH - 2, nrow - 1 - t));                           \
-    if (t + LR_DEPTH - 1 < nrow) ring_issue<W, NT, EK>(a, sc, st, ring, wave, t + LR_DEPTH - 1); \
+    if (t + LR_DEPTH - 1 < nrow) issue(t + LR_DEPTH - 1);                            \
     if (any) {                                                                       \
       RowIn<T, NT> in;                                                               \
       ring_read<W, NT, QN>(ring, wave, lane, t, in);                                 \
