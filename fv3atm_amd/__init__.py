@@ -205,8 +205,8 @@ def step_timing(ctx: Context):
 
 
 def set_concurrency(ctx: Context, on: bool = True):
-    """Edge kernels of the fused transport beside the lean kernel on two
-    auxiliary streams (True, the default) or after it on the step's stream."""
+    """Edge kernels of the fused transport beside the lean kernel on auxiliary
+    streams (True, the default) or after it on the step's stream."""
     check(_ffi.lib().fv3_ctx_set_concurrency(ctx.handle, 1 if on else 0), "fv3_ctx_set_concurrency")
 
 

@@ -187,7 +187,8 @@ int fv3_ctx_create(const fv3_bounds *bd, int npx, int npy, int npz, int ntiles, 
     }
   if (hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c.join_ev[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c.join_ev[1], hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c.join_ev[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.join_ev[2], hipEventDisableTiming) != hipSuccess) {
     fv3_ctx_destroy(h);
     return set_error("hipEventCreateWithFlags failed");
   }

@@ -9,9 +9,10 @@ namespace fv3 {
 // launch configurations (waves per workgroup x tracers per wave); cfg 0 is
 // the default, the others are the autotuner's candidates
 constexpr int FUSED32_NCFG = 15;
-// streams of the three kernel kinds (lean, edge rows, edge strips)
+// streams of the kernel kinds (lean, edge rows, edge strips / left strips,
+// right strips)
 struct KindStreams {
-  hipStream_t k[3];
+  hipStream_t k[4];
 };
 int launch_fused_f64(const FusedArgs<double> &a, const KindStreams &s);
 int launch_fused_f32(const FusedArgs<float> &a, int cfg, const KindStreams &s);

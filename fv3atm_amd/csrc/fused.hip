@@ -13,12 +13,14 @@ namespace fv3 {
 template <int HO>
 static void go64(const FusedArgs<double> &a, const KindStreams &s) {
   const dim3 b(FT_LANES * 4);
-  int n0, n1, n2;
-  fused_grid_split(a, n0, n1, n2);
-  auto grid = [&](int n) { return dim3((unsigned)n, (unsigned)a.npz, (unsigned)a.ntiles); };
-  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 0, 1>), grid(n0), b, 0, s.k[0], a);
-  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 1, 1>), grid(n1), b, 0, s.k[1], a);
-  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 2, 1>), grid(n2), b, 0, s.k[2], a);
+  int n[5];
+  fused_grid_split(a, n);
+  auto grid = [&](int c) { return dim3((unsigned)c, (unsigned)a.npz, (unsigned)a.ntiles); };
+  if (n[0] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 0, 1>), grid(n[0]), b, 0, s.k[0], a);
+  if (n[1] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 1, 1>), grid(n[1]), b, 0, s.k[1], a);
+  if (n[2] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 2, 1>), grid(n[2]), b, 0, s.k[2], a);
+  if (n[3] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 3, 1>), grid(n[3]), b, 0, s.k[2], a);
+  if (n[4] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 4, 1>), grid(n[4]), b, 0, s.k[3], a);
 }
 
 int launch_fused_f64(const FusedArgs<double> &a, const KindStreams &s) {
@@ -156,14 +158,13 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   if (timing) FV3_HIP(hipEventRecord(c.ev[0], s));
   // each pass: the edge kinds fork onto the auxiliary streams and join back
   // before the next pass (which reads every face's previous pass)
-  const bool conc = c.concurrent_kinds && c.aux[0] && c.aux[1];
-  const KindStreams ks{{s, conc ? c.aux[0] : s, conc ? c.aux[1] : s}};
+  const bool conc = c.concurrent_kinds && c.aux[0] && c.aux[1] && c.aux[2];
+  const KindStreams ks{{s, conc ? c.aux[0] : s, conc ? c.aux[1] : s, conc ? c.aux[2] : s}};
   for (int it = 1; it <= cap; it++) {
     a.it = it;
     if (conc) {
       FV3_HIP(hipEventRecord(c.fork_ev, s));
-      FV3_HIP(hipStreamWaitEvent(c.aux[0], c.fork_ev, 0));
-      FV3_HIP(hipStreamWaitEvent(c.aux[1], c.fork_ev, 0));
+      for (int i = 0; i < 3; i++) FV3_HIP(hipStreamWaitEvent(c.aux[i], c.fork_ev, 0));
     }
     int rc;
     if constexpr (std::is_same<T, double>::value)
@@ -171,12 +172,11 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     else
       rc = launch_fused_f32(a, c.fused_cfg, ks);
     if (rc) return rc;
-    if (conc) {
-      FV3_HIP(hipEventRecord(c.join_ev[0], c.aux[0]));
-      FV3_HIP(hipEventRecord(c.join_ev[1], c.aux[1]));
-      FV3_HIP(hipStreamWaitEvent(s, c.join_ev[0], 0));
-      FV3_HIP(hipStreamWaitEvent(s, c.join_ev[1], 0));
-    }
+    if (conc)
+      for (int i = 0; i < 3; i++) {
+        FV3_HIP(hipEventRecord(c.join_ev[i], c.aux[i]));
+        FV3_HIP(hipStreamWaitEvent(s, c.join_ev[i], 0));
+      }
   }
   if (nsplit_cap > 0) {
     hipLaunchKernelGGL(k_nsplit_check, dim3(1), dim3(1), 0, s, nspl, npz, cap, flag);

@@ -12,12 +12,15 @@ namespace fv3 {
 template <int W, int NT, int HO, int OCC_E, int WE, int NTE, int OCC_L0>
 static void go2(const FusedArgs<float> &a, const KindStreams &s) {
   constexpr int OCC_L = OCC_L0 > 0 ? OCC_L0 : (NT > 1 ? 1 : (W >= 4 ? 12 / W : 3));
-  int n0, n1, n2;
-  fused_grid_split(a, n0, n1, n2);
-  auto grid = [&](int n) { return dim3((unsigned)n, (unsigned)a.npz, (unsigned)a.ntiles); };
-  if (n0 > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 0, OCC_L>), grid(n0), dim3(FT_LANES * W), 0, s.k[0], a);
-  if (n1 > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 1, OCC_E>), grid(n1), dim3(FT_LANES * WE), 0, s.k[1], a);
-  if (n2 > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 2, OCC_E>), grid(n2), dim3(FT_LANES * WE), 0, s.k[2], a);
+  int n[5];
+  fused_grid_split(a, n);
+  auto grid = [&](int c) { return dim3((unsigned)c, (unsigned)a.npz, (unsigned)a.ntiles); };
+  const dim3 bl(FT_LANES * W), be(FT_LANES * WE);
+  if (n[0] > 0) hipLaunchKernelGGL((k_fused_tp<float, W, NT, HO, 0, OCC_L>), grid(n[0]), bl, 0, s.k[0], a);
+  if (n[1] > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 1, OCC_E>), grid(n[1]), be, 0, s.k[1], a);
+  if (n[2] > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 2, OCC_E>), grid(n[2]), be, 0, s.k[2], a);
+  if (n[3] > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 3, OCC_E>), grid(n[3]), be, 0, s.k[2], a);
+  if (n[4] > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 4, OCC_E>), grid(n[4]), be, 0, s.k[3], a);
 }
 
 template <int W, int NT, int OCC_E = 1, int WE = W, int NTE = NT, int OCC_L = 0>

@@ -90,7 +90,7 @@ __device__ __forceinline__ void q_src(const FusedArgs<T> &a, const StripCtx<T> &
                                       const RowIdx<T> &x, const T *&py, const T *&px) {
   const int nid = a.nid;
   // EK 1 blocks have every lane inside the compute domain, EK 0 every cell
-  const bool inside = EK == 0 || ((EK == 1 || sc.lane_in) && x.row_in);
+  const bool inside = EK == 0 || ((EK == 1 || sc.lane_in) && x.row_in);  // EK >= 2: strips
   py = (inside ? s.qin : s.qhalo) + x.orow + sc.lo_q;
   px = py;
   if constexpr (EK == 1) {
@@ -101,7 +101,7 @@ __device__ __forceinline__ void q_src(const FusedArgs<T> &a, const StripCtx<T> &
       px = py;
     }
   }
-  if constexpr (EK == 2) {
+  if constexpr (EK >= 2) {
     if (a.ntiles > 1 && !inside) {
       const int n = a.ie;
       const int tile = blockIdx.z + 1;
@@ -186,7 +186,7 @@ __device__ __forceinline__ void row_fetch(const FusedArgs<T> &a, const StripCtx<
 
 // One input row r = j0 - 3 + t of the streaming pass; U = t mod 6.  HO: the
 // advection order when fixed at compile time (0: taken from the arguments).
-template <typename T, int NT, int U, int HO>
+template <typename T, int NT, int U, int HO, int SIDE = 0>
 __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T> &sc,
                                          LvState<T> &lv, TrState<T> (&st)[NT], const bool (&act)[NT],
                                          int t, const RowIn<T, NT> &in) {
@@ -285,8 +285,8 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     else
       q_i = (qm * area_m + lv.yfx_prev * s.fy2_prev - yfx * fy2) * rray;
     // outer x-sweep of row r-3, inner x-sweep of row r, q_j of row r
-    const T fx = xsweep(q_i, cxm, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, hord, dxa_m);
-    const T fx2 = xsweep(qx, cxr, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, ord_in, dxa_r);
+    const T fx = xsweep<SIDE>(q_i, cxm, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, hord, dxa_m);
+    const T fx2 = xsweep<SIDE>(qx, cxr, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, ord_in, dxa_r);
     s.fx2_r[U] = fx2;
     const T fx2p = from_right(fx2);
     T q_j;
@@ -342,8 +342,8 @@ __device__ __forceinline__ void wait_vm_lgkm0() {
 template <int W, int NT, int EK>
 __device__ __forceinline__ void ring_issue(const FusedArgs<float> &a, const StripCtx<float> &sc,
                                            const TrState<float> (&st)[NT],
-                                           LdsRing<W, NT, EK == 2 ? 2 : 1> &ring, int wave, int t) {
-  constexpr int QN = EK == 2 ? 2 : 1;
+                                           LdsRing<W, NT, EK >= 2 ? 2 : 1> &ring, int wave, int t) {
+  constexpr int QN = EK >= 2 ? 2 : 1;
   const RowIdx<float> x = row_idx(a, sc, t);
   const int nid = a.nid;
   float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
@@ -484,7 +484,9 @@ __device__ __forceinline__ void ring_wait(int ahead) {
 // EK (edge kind) 0: the lean variant for the interior blocks (no cube-edge,
 // corner or halo code is compiled); 1: blocks of the interior strips that
 // reach a halo or cube-edge row (no per-lane edge or corner code); 2: the
-// strips that reach a halo or cube-edge column.
+// strips that reach a halo or cube-edge column when no interior strip
+// exists; 3 / 4: those strips left / right of the interior strips (each can
+// only meet its own cube edge).
 // OCC: minimum workgroups per CU asked of the register allocator.
 template <typename T, int W, int NT, int HO, int EK, int OCC>
 __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T> a) {
@@ -508,11 +510,18 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
       strip = a.s_lo + bx % ns;
       vb = bx / ns;
       if (vb >= a.v_lo) vb += nv;
-    } else {  // strips outside [s_lo, s_hi], every row block
+    } else if constexpr (EK == 2) {  // strips outside [s_lo, s_hi], every row block
       const int nso = a.nstrip - ns;
       strip = bx % nso;
       if (ns > 0 && strip >= a.s_lo) strip += ns;
       vb = bx / nso;
+    } else if constexpr (EK == 3) {  // strips left of the interior strips
+      strip = bx % a.s_lo;
+      vb = bx / a.s_lo;
+    } else {  // strips right of the interior strips
+      const int nr = a.nstrip - 1 - a.s_hi;
+      strip = a.s_hi + 1 + bx % nr;
+      vb = bx / nr;
     }
   }
   const int i0 = a.is + strip * FT_BX;
@@ -526,9 +535,9 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
   sc.lo_mx = (sc.i > a.ie ? a.ie : (sc.i < a.is ? a.is : sc.i)) - a.is;
   sc.col_out = lane >= NG && lane < NG + FT_BX && sc.i <= a.ie;
   sc.lane_in = sc.i >= a.is && sc.i <= a.ie;
-  sc.edge_strip = EK == 2 && a.edge && (i0 - NG <= 2 || i0 - NG + FT_LANES - 1 >= a.npx - 2);
+  sc.edge_strip = EK >= 2 && a.edge && (i0 - NG <= 2 || i0 - NG + FT_LANES - 1 >= a.npx - 2);
   sc.edge_rows = EK != 0;
-  sc.corner_wg = EK == 2 && a.corners != 0u && (strip == 0 || strip == a.nstrip - 1) && (vb == 0 || vb == a.nvblk - 1);
+  sc.corner_wg = EK >= 2 && a.corners != 0u && (strip == 0 || strip == a.nstrip - 1) && (vb == 0 || vb == a.nvblk - 1);
   sc.scale = nspl > 1;
   sc.frac = T(1) / T(nspl);
   sc.nix = a.ie - a.is + 2;
@@ -570,7 +579,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
     lv.yfx_prev = lv.mfy_prev = T(0);
     const bool any = act[0];
     if constexpr (sizeof(T) == 4) {
-      constexpr int QN = EK == 2 ? 2 : 1;
+      constexpr int QN = EK >= 2 ? 2 : 1;
       static_assert(LR_DEPTH == 4, "ring_wait covers up to two rows ahead");
       __shared__ LdsRing<W, NT, QN> ring;
       // the previous round's last reads of every slot are done before the
@@ -594,7 +603,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
     if (any) {                                                                       \
       RowIn<T, NT> in;                                                               \
       ring_read<W, NT, QN>(ring, wave, lane, t, in);                                 \
-      row_step<T, NT, U, HO>(a, sc, lv, st, act, t, in);                             \
+      row_step<T, NT, U, HO, EK == 3 ? 1 : (EK == 4 ? 2 : 0)>(a, sc, lv, st, act, t, in); \
     }                                                                                \
   }
         FV3_RROW(0) FV3_RROW(1) FV3_RROW(2) FV3_RROW(3) FV3_RROW(4) FV3_RROW(5)
@@ -612,7 +621,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
 #define FV3_ROW(U)                                                                   \
   if (any && t0 + U < nrow) {                                                        \
     if (t0 + U + 1 < nrow) row_fetch<T, NT>(a, sc, st, act, t0 + U + 1, buf[(U + 1) & 1]); \
-    row_step<T, NT, U, HO>(a, sc, lv, st, act, t0 + U, buf[U & 1]);                  \
+    row_step<T, NT, U, HO, EK == 3 ? 1 : (EK == 4 ? 2 : 0)>(a, sc, lv, st, act, t0 + U, buf[U & 1]); \
   }                                                                                  \
   if (W > 1) __builtin_amdgcn_s_barrier();
       FV3_ROW(0) FV3_ROW(1) FV3_ROW(2) FV3_ROW(3) FV3_ROW(4) FV3_ROW(5)

@@ -62,15 +62,19 @@ struct FusedArgs {
   long long ms2;                   // metric slab (isd:ied, jsd:jed)
 };
 
-// Block counts of the three kernel kinds over (strip, row block): lean
-// interior blocks, edge-row blocks of the interior strips, edge strips.
+// Block counts of the kernel kinds over (strip, row block): n[0] lean
+// interior blocks, n[1] edge-row blocks of the interior strips, n[2] edge
+// strips when there is no interior strip, n[3] / n[4] the strips left / right
+// of the interior strips.
 template <typename T>
-inline void fused_grid_split(const FusedArgs<T> &a, int &n_lean, int &n_rows, int &n_strips) {
+inline void fused_grid_split(const FusedArgs<T> &a, int (&n)[5]) {
   const int ns = a.s_hi >= a.s_lo ? a.s_hi - a.s_lo + 1 : 0;
   const int nv = a.v_hi >= a.v_lo ? a.v_hi - a.v_lo + 1 : 0;
-  n_lean = ns * nv;
-  n_rows = ns * (a.nvblk - nv);
-  n_strips = (a.nstrip - ns) * a.nvblk;
+  n[0] = ns * nv;
+  n[1] = ns * (a.nvblk - nv);
+  n[2] = ns > 0 ? 0 : a.nstrip * a.nvblk;
+  n[3] = ns > 0 ? a.s_lo * a.nvblk : 0;
+  n[4] = ns > 0 ? (a.nstrip - 1 - a.s_hi) * a.nvblk : 0;
 }
 
 // ---- cross-lane shifts: value of lane l-1 / l+1 (DPP wave_shr:1 / wave_shl:1)
@@ -199,6 +203,38 @@ __device__ __forceinline__ void edge_coef_sel(T qm1, T q0, T qp1, T dmm1, T dm0,
   }
 }
 
+// The same for a strip that can only touch the left cube edge (roles 1..3,
+// SIDE 1) or only the right one (roles 4..6, SIDE 2): half the candidates.
+template <int SIDE, typename T>
+__device__ __forceinline__ void edge_coef_side(T qm1, T q0, T qp1, T dmm1, T dm0, T dmp1, T al0, T alp1,
+                                               int role, T efl, T efr, T &bl, T &br) {
+  T l, r;
+  if constexpr (SIDE == 1) {
+    const T b1 = K<T>::s14 * dmm1 + K<T>::s11 * (qm1 - q0);
+    const T b3 = K<T>::s15 * qm1 + K<T>::s11 * q0 - K<T>::s14 * dm0 - q0;
+    const T bL = efl - q0;  // role 2
+    const T r2 = K<T>::s15 * q0 + K<T>::s11 * qp1 - K<T>::s14 * dmp1 - q0;
+    const T r3 = alp1 - q0;
+    const T rR = efr - q0;  // role 1
+    l = role == 1 ? b1 : (role == 3 ? b3 : bL);
+    r = role == 2 ? r2 : (role == 3 ? r3 : rR);
+  } else {
+    const T b4 = al0 - q0;
+    const T b5 = K<T>::s15 * q0 + K<T>::s11 * qm1 + K<T>::s14 * dmm1 - q0;
+    const T bL = efl - q0;  // role 6
+    const T r4 = K<T>::s15 * qp1 + K<T>::s11 * q0 + K<T>::s14 * dm0 - q0;
+    const T r6 = K<T>::s11 * (qp1 - q0) - K<T>::s14 * dmp1;
+    const T rR = efr - q0;  // role 5
+    l = role == 4 ? b4 : (role == 5 ? b5 : bL);
+    r = role == 4 ? r4 : (role == 6 ? r6 : rR);
+  }
+  pert_ppm1(q0, l, r, 1);
+  if (role != 0) {
+    bl = l;
+    br = r;
+  }
+}
+
 // (bl, br) of an interior cell for the monotone orders
 template <typename T>
 __device__ __forceinline__ void cell_coef(T qm2, T qm1, T q0, T qp1, T qp2, T dmm1, T dm0, T dmp1,
@@ -244,7 +280,9 @@ __device__ __forceinline__ T face_flux2(T c, T qL, T blL, T brL, T qR, T blR, T 
 
 // x-direction PPM across the lanes of a row: flux at the face left of each
 // lane's cell.  dxl: dxa of the lane's cell (used in edge strips only).
-template <typename T>
+// SIDE: 0 a strip that may touch either cube edge, 1 only the left one, 2
+// only the right one (the edge code then evaluates half the candidates).
+template <int SIDE = 0, typename T>
 __device__ __forceinline__ T xsweep(T q, T c, int i, int is, int ie, int npx, bool edge,
                                     bool edge_strip, int iord, T dxl) {
   const T qm1 = from_left(q), qp1 = from_right(q);
@@ -264,7 +302,10 @@ __device__ __forceinline__ T xsweep(T q, T c, int i, int is, int ie, int npx, bo
     const int role = edge_role(i, is, ie, npx, edge);
     const T d1 = from_left(dxl), d3 = from_right(dxl), d4 = from_right(d3);
     const T efr = edge_face(qm1, q, qp1, qp2, d1, dxl, d3, d4);
-    edge_coef_sel(qm1, q, qp1, dmm1, dm, dmp1, al, alp1, role, from_left(efr), efr, bl, br);
+    if constexpr (SIDE == 0)
+      edge_coef_sel(qm1, q, qp1, dmm1, dm, dmp1, al, alp1, role, from_left(efr), efr, bl, br);
+    else
+      edge_coef_side<SIDE>(qm1, q, qp1, dmm1, dm, dmp1, al, alp1, role, from_left(efr), efr, bl, br);
   }
   const T blm = from_left(bl), brm = from_left(br);
   return face_flux2(c, qm1, blm, brm, q, bl, br);

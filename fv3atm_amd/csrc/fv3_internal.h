@@ -80,12 +80,12 @@ struct Ctx {
   bool graph_safe = false;
   int *step_flag = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  // the fused transport's edge kinds run beside the lean kind on two
+  // the fused transport's edge kinds run beside the lean kind on three
   // auxiliary streams (default priority), forked from and joined to the
   // step's stream with events every pass
   bool concurrent_kinds = true;
-  hipStream_t aux[2] = {nullptr, nullptr};
-  hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
+  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
   int last_passes = 0;
   bool ev_valid = false;
 };

@@ -144,7 +144,7 @@ int fv3_ctx_set_timing(fv3_ctx *ctx, int on);
 int fv3_ctx_set_launch(fv3_ctx *ctx, int cfg, int rows, int remap);
 /* No counterpart upstream: run the fused transport's two edge kernels (the
  * blocks that reach a halo or cube-edge row or column) beside the lean
- * interior kernel on two auxiliary streams of the context, forked from and
+ * interior kernel on auxiliary streams of the context, forked from and
  * joined to the step's stream every pass (on = 1, the default), or after it
  * on the step's stream (on = 0).  Results are identical either way. */
 int fv3_ctx_set_concurrency(fv3_ctx *ctx, int on);
