@@ -113,10 +113,11 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   a.H = std::min(nj, c.fused_h > 0 ? c.fused_h : 60);
   a.nstrip = (ni + FT_BX - 1) / FT_BX;
   a.nvblk = (nj + a.H - 1) / a.H;
-  // interior blocks: every lane's column i0-3..i0+60 and every input row
-  // j0-3..j0+H+2 lie in the compute domain (no halo lane or row) and, on a
-  // cube face, every stencil cell is 3 or more cells inside the tile (no
-  // cube-edge role, no corner)
+  // interior strips: every lane's column i0-3..i0+60 lies in the compute
+  // domain and, on a cube face, 3 or more cells inside the tile; their lean
+  // rows lt..lb keep every input row j0-3..j0+H+2 in the domain and every
+  // stencil row off the cube-edge rows, so the lean kernel needs no halo,
+  // corner or cube-edge code
   {
     const bool edges = d.edge || (!d.bounded && d.corners != 0u);
     auto strip_ok = [&](int st) {
@@ -124,17 +125,19 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
       const int lo = i0 - NG, hi = i0 - NG + FT_LANES - 1;
       return lo >= d.is && hi <= d.ie && (!edges || (lo >= 3 && hi <= d.npx - 3));
     };
-    auto vb_ok = [&](int v) {
-      const int j0 = d.js + v * a.H;
-      return j0 - NG >= d.js && j0 + a.H + 2 <= d.je && (!edges || (j0 - 5 >= 3 && j0 + a.H <= d.npy - 3));
-    };
-    a.s_lo = 1; a.s_hi = 0; a.v_lo = 1; a.v_hi = 0;
-    int s0 = -1, s1 = -2, v0 = -1, v1 = -2;
+    // (js + 6: the output row of the first input row, three above the first
+    // lean row, is itself a domain row, so no row index of a lean block clamps)
+    a.lt = std::max(d.js + 2 * NG, edges ? 8 : d.js + 2 * NG);
+    a.lb = std::min(d.je - NG, edges ? d.npy - 4 : d.je - NG);
+    a.s_lo = 1; a.s_hi = 0; a.nmid = 0;
+    int s0 = -1, s1 = -2;
     for (int st = 0; st < a.nstrip; st++)
       if (strip_ok(st)) { if (s0 < 0) s0 = st; s1 = st; }
-    for (int v = 0; v < a.nvblk; v++)
-      if (vb_ok(v)) { if (v0 < 0) v0 = v; v1 = v; }
-    if (s0 >= 0 && v0 >= 0) { a.s_lo = s0; a.s_hi = s1; a.v_lo = v0; a.v_hi = v1; }
+    if (s0 >= 0 && a.lb >= a.lt) {
+      a.s_lo = s0;
+      a.s_hi = s1;
+      a.nmid = (a.lb - a.lt + a.H) / a.H;
+    }
   }
   a.nsplit = nspl;
   a.q_in = q_in;

@@ -17,7 +17,7 @@ __device__ __forceinline__ void pass_buffers(const FusedArgs<T> &a, int nspl, co
 // per (strip, block, level) constants of a wave
 template <typename T>
 struct StripCtx {
-  int i, ic, j0, nix, niy, lo_q, lo_fx, lo_mx;  // lane column; lane offsets in the row layouts
+  int i, ic, j0, h, nix, niy, lo_q, lo_fx, lo_mx;  // lane column; block rows; lane offsets in the row layouts
   bool col_out, lane_in, edge_strip, edge_rows, corner_wg, scale;
   T frac;
   const T *cxk, *cyk, *mfxk, *mfyk;
@@ -74,7 +74,7 @@ __device__ __forceinline__ RowIdx<T> row_idx(const FusedArgs<T> &a, const StripC
   x.fc = f < a.js ? a.js : (f > a.je + 1 ? a.je + 1 : f);
   x.orow = (long long)(x.rc - a.jsd) * a.nid;
   x.row_in = x.rc >= a.js && x.rc <= a.je;
-  x.out_row = t >= 2 * NG && x.m >= sc.j0 && x.m < sc.j0 + a.H && x.m <= a.je;
+  x.out_row = t >= 2 * NG && x.m >= sc.j0 && x.m < sc.j0 + sc.h && x.m <= a.je;
   x.corow = (long long)(x.mc - a.jsd) * a.nid;
   x.co = x.corow + sc.lo_q;
   return x;
@@ -498,35 +498,41 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
   // whole block: this level is done, or needs more passes than are launched
   // (reported by the caller's nsplit check; nothing is written for it)
   if (a.it > nspl || nspl > a.cap) return;
-  int strip, vb;
+  int strip, vb, j0, h;
   {
     const int ns = a.s_hi >= a.s_lo ? a.s_hi - a.s_lo + 1 : 0;
-    const int nv = a.v_hi >= a.v_lo ? a.v_hi - a.v_lo + 1 : 0;
     const int bx = (int)blockIdx.x;
-    if constexpr (EK == 0) {
-      strip = a.s_lo + bx % ns;
-      vb = a.v_lo + bx / ns;
-    } else if constexpr (EK == 1) {  // interior strips, row blocks outside [v_lo, v_hi]
+    if constexpr (EK == 0) {  // interior strips, lean rows
       strip = a.s_lo + bx % ns;
       vb = bx / ns;
-      if (vb >= a.v_lo) vb += nv;
-    } else if constexpr (EK == 2) {  // strips outside [s_lo, s_hi], every row block
-      const int nso = a.nstrip - ns;
-      strip = bx % nso;
-      if (ns > 0 && strip >= a.s_lo) strip += ns;
-      vb = bx / nso;
-    } else if constexpr (EK == 3) {  // strips left of the interior strips
-      strip = bx % a.s_lo;
-      vb = bx / a.s_lo;
-    } else {  // strips right of the interior strips
-      const int nr = a.nstrip - 1 - a.s_hi;
-      strip = a.s_hi + 1 + bx % nr;
-      vb = bx / nr;
+      j0 = a.lt + vb * a.H;
+      h = min(a.H, a.lb + 1 - j0);
+    } else if constexpr (EK == 1) {  // interior strips, the rows above and below the lean rows
+      strip = a.s_lo + bx % ns;
+      vb = bx / ns;
+      j0 = vb == 0 ? a.js : a.lb + 1;
+      h = vb == 0 ? a.lt - a.js : a.je - a.lb;
+    } else {
+      if constexpr (EK == 2) {  // every strip (no interior strip exists)
+        strip = bx % a.nstrip;
+        vb = bx / a.nstrip;
+      } else if constexpr (EK == 3) {  // strips left of the interior strips
+        strip = bx % a.s_lo;
+        vb = bx / a.s_lo;
+      } else {  // strips right of the interior strips
+        const int nr = a.nstrip - 1 - a.s_hi;
+        strip = a.s_hi + 1 + bx % nr;
+        vb = bx / nr;
+      }
+      j0 = a.js + vb * a.H;
+      h = min(a.H, a.je + 1 - j0);
     }
+    (void)ns;
   }
   const int i0 = a.is + strip * FT_BX;
   StripCtx<T> sc;
-  sc.j0 = a.js + vb * a.H;
+  sc.j0 = j0;
+  sc.h = h;
   sc.i = i0 - NG + lane;
   sc.ic = sc.i > a.ied ? a.ied : sc.i;
   sc.lo_q = sc.ic - a.isd;
@@ -559,7 +565,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
   pass_buffers(a, nspl, in, out);
   const int per_round = W * NT;
   const int rounds = (a.nq + per_round - 1) / per_round;
-  const int nrow = a.H + 2 * NG;
+  const int nrow = sc.h + 2 * NG;
   for (int rd = 0; rd < rounds; rd++) {
     TrState<T> st[NT];
     bool act[NT];

@@ -42,9 +42,12 @@ struct FusedArgs {
   unsigned corners;
   bool edge;
   int npz, nq, hord, it, cap, H, nstrip, nvblk;
-  // blocks (strip, vblock) inside [s_lo, s_hi] x [v_lo, v_hi] touch no
-  // cube-edge cell, corner or halo lane: the lean kernel variant takes them
-  int s_lo, s_hi, v_lo, v_hi;
+  // Interior strips [s_lo, s_hi] touch no cube-edge column or halo lane.
+  // Their rows lt..lb touch no cube-edge row or halo row either: the lean
+  // kernel variant takes them in nmid blocks of H rows; the edge-row variant
+  // takes rows js..lt-1 and lb+1..je.  Every other strip is cut into nvblk
+  // blocks of H rows.
+  int s_lo, s_hi, lt, lb, nmid;
   // six tiles (blockIdx.z = tile): per-tile strides of q, winds, dp1, the
   // metrics and the FP32 factors; halo cells are read from the neighbour
   // tile's interior in the pass input buffer
@@ -69,9 +72,8 @@ struct FusedArgs {
 template <typename T>
 inline void fused_grid_split(const FusedArgs<T> &a, int (&n)[5]) {
   const int ns = a.s_hi >= a.s_lo ? a.s_hi - a.s_lo + 1 : 0;
-  const int nv = a.v_hi >= a.v_lo ? a.v_hi - a.v_lo + 1 : 0;
-  n[0] = ns * nv;
-  n[1] = ns * (a.nvblk - nv);
+  n[0] = ns * a.nmid;
+  n[1] = ns * 2;  // the top and bottom edge-row blocks of each interior strip
   n[2] = ns > 0 ? 0 : a.nstrip * a.nvblk;
   n[3] = ns > 0 ? a.s_lo * a.nvblk : 0;
   n[4] = ns > 0 ? (a.nstrip - 1 - a.s_hi) * a.nvblk : 0;
