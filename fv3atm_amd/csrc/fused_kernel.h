@@ -441,6 +441,25 @@ __device__ __forceinline__ void ring_issue_lean(const FusedArgs<float> &a, const
     lds_dma(st[n].qin + orow + sc.lo_q, slot[LR_NL + wave * NT + n]);
 }
 
+template <int W, int NT, int EK>
+__device__ __forceinline__ void ring_issue_mixed(const FusedArgs<float> &a, const StripCtx<float> &sc,
+                                                 const TrState<float> (&st)[NT], const LeanRows<W> &lr,
+                                                 LdsRing<W, NT, EK >= 2 ? 2 : 1> &ring, int wave, int t) {
+  constexpr int QN = EK >= 2 ? 2 : 1;
+  float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
+#pragma unroll
+  for (int m = 0; m < LeanRows<W>::M; m++)
+    if (wave + m * W < LR_NL) lds_dma(lr.p[m] + (long long)t * lr.step[m] + lr.loff[m], slot[wave + m * W]);
+  const RowIdx<float> x = row_idx(a, sc, t);
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    const float *py, *px;
+    q_src<float, EK>(a, sc, st[n], x, py, px);
+    lds_dma(py, slot[LR_NL + (wave * NT + n) * QN]);
+    if (QN == 2) lds_dma(px, slot[LR_NL + (wave * NT + n) * QN + 1]);
+  }
+}
+
 template <int W, int NT, int QN>
 __device__ __forceinline__ void ring_read(const LdsRing<W, NT, QN> &ring, int wave, int lane, int t,
                                           RowIn<float, NT> &in) {
@@ -591,13 +610,19 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
       // the previous round's last reads of every slot are done before the
       // prologue overwrites them
       if (rd > 0) ring_wait<W, NT, QN>(0);
-      LeanRows<W> lr;
-      if constexpr (EK == 0) lr = lean_rows<W>(a, sc, wave);
+      const LeanRows<W> lr = lean_rows<W>(a, sc, wave);
       auto issue = [&](int t) {
-        if constexpr (EK == 0)
+        if constexpr (EK == 0) {
           ring_issue_lean<W, NT>(a, sc, st, lr, ring, wave, t);
-        else
-          ring_issue<W, NT, EK>(a, sc, st, ring, wave, t);
+        } else {
+          // rows whose row indices do not clamp take the affine addresses
+          // for the level arrays; the q sources keep their per-lane rules
+          const int r = sc.j0 - NG + t;
+          if (r >= a.js + NG && r <= a.je)
+            ring_issue_mixed<W, NT, EK>(a, sc, st, lr, ring, wave, t);
+          else
+            ring_issue<W, NT, EK>(a, sc, st, ring, wave, t);
+        }
       };
       for (int t = 0; t < LR_DEPTH - 1 && t < nrow; t++) issue(t);
       for (int t0 = 0; t0 < nrow; t0 += 6) {
