@@ -213,12 +213,13 @@ __device__ __forceinline__ void cell_limit(int c, int km, int iv, int akord, flo
 }
 
 // Mapping state of one (column, field): the open target kk (whose bottom
-// interface has been passed), its partial sum, and the rows of targets kk and
-// kk-1 (prefetched one ahead).
+// interface has been passed), its partial sum, and the row of target kk.
+// The row of the next target is loaded as soon as kk closes; it is first
+// needed a whole source layer later, which hides the load.
 struct MapState {
   int kk;
   float acc;
-  TargetRow cur, nxt;
+  TargetRow cur;
 };
 
 __device__ __forceinline__ void map_layer(MapState &m, int c, int km, float a1, float a2, float a3,
@@ -239,8 +240,7 @@ __device__ __forceinline__ void map_layer(MapState &m, int c, int km, float a1, 
     bot_in = true;
     sb = s;
     m.kk--;
-    m.cur = m.nxt;
-    if (m.kk >= 2) m.nxt = load_target(o, m.kk - 1, col, nc);
+    if (m.kk >= 1) m.cur = load_target(o, m.kk, col, nc);
   }
   if (m.kk >= 1) {
     if (!bot_in)
@@ -314,7 +314,6 @@ __global__ void __launch_bounds__(RF_COLS * RF_WAVES) k_remap_fields(const Field
   ms.kk = km;
   ms.acc = 0.f;
   ms.cur = load_target(o, km, col, nc);
-  ms.nxt = load_target(o, km - 1, col, nc);
 
   for (int b = a.nb - 1; b >= 0; b--) {
     const int k0 = b * B + 1;
