@@ -7,6 +7,10 @@
       HBM bytes of every dispatch whose name contains K (FETCH_SIZE run plus
       WRITE_SIZE run, each calibrated as below), summed and divided by L
       launches; prints a 'bench-traffic:' line carrying the tag
+  python -m fv3atm_amd.rocprof_summary spans DIR --kernel K --group G
+      wall span (first start to last end) of every G consecutive dispatches
+      whose name contains K, ordered by start: one launch made of G kernels
+      that may run concurrently (mean, min, max)
   python -m fv3atm_amd.rocprof_summary counters DIR [--calib-bytes B]
       per-kernel mean of every collected counter per dispatch; with
       --calib-bytes, the counter of the calibration kernel (the one whose
@@ -75,6 +79,18 @@ def kernel_stats(d):
     return agg
 
 
+def spans(d, kernel, group):
+    """Wall spans (ns) of consecutive groups of `group` dispatches of kernels
+    whose short name contains `kernel`, in start order."""
+    ev = sorted((float(_col(r, "Start_Timestamp")), float(_col(r, "End_Timestamp")))
+                for r in _rows(d, "kernel_trace.csv") if kernel in short(_col(r, "Kernel_Name")))
+    out = []
+    for i in range(0, len(ev) - group + 1, group):
+        g = ev[i:i + group]
+        out.append(max(e for _, e in g) - min(b for b, _ in g))
+    return out
+
+
 def counter_means(d):
     """{(kernel, counter): [dispatches, sum]} over dispatches (a counter that a
     dispatch reports once per agent/dimension is summed within the dispatch)."""
@@ -112,12 +128,13 @@ def calibrated_sum(d, counter, kernel, calib_name, calib_bytes):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "counters", "traffic"])
+    ap.add_argument("mode", choices=["stats", "counters", "traffic", "spans"])
     ap.add_argument("dir")
     ap.add_argument("dir2", nargs="?")
     ap.add_argument("--kernel", default="k_fused_tp")
     ap.add_argument("--launches", type=int, default=1)
     ap.add_argument("--tag", default="")
+    ap.add_argument("--group", type=int, default=1)
     ap.add_argument("--calib-bytes", type=float, default=0.0)
     ap.add_argument("--calib-name", default="bitwise_not")
     ap.add_argument("--top", type=int, default=12)
@@ -129,6 +146,13 @@ def main(argv=None):
         w("| kernel | calls | total ms | mean us | share |\n|---|---:|---:|---:|---:|\n")
         for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[: a.top]:
             w(f"| `{clip(n)}` | {c} | {t / 1e6:.3f} | {t / c / 1e3:.1f} | {100 * t / tot:.1f}% |\n")
+        return
+    if a.mode == "spans":
+        sp = spans(a.dir, a.kernel, a.group)
+        if not sp:
+            raise SystemExit(f"no dispatches of {a.kernel}")
+        w(f"{len(sp)} launches of {a.group} `{a.kernel}` kernels: span mean {sum(sp) / len(sp) / 1e6:.3f} ms, "
+          f"min {min(sp) / 1e6:.3f} ms, max {max(sp) / 1e6:.3f} ms\n")
         return
     if a.mode == "traffic":
         f, nf, ff = calibrated_sum(a.dir, "FETCH_SIZE", a.kernel, a.calib_name, a.calib_bytes)

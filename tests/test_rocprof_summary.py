@@ -60,3 +60,14 @@ def test_traffic_line_and_bench_lookup(tmp_path):
     # fetch: (10 + 20) KiB x 2 = 61440 B; write: (3 + 5) KiB x 1 = 8192 B; per launch 34816 B
     line = [l for l in out.getvalue().splitlines() if l.startswith("bench-traffic:")][0]
     assert line == "bench-traffic: n=8 npz=2 nq=1 cfg=0 rows=60 bytes_per_pass=34816"
+
+
+def test_spans_of_concurrent_groups(tmp_path):
+    d = tmp_path / "t"
+    d.mkdir()
+    k = "void fv3::k_fused_tp<float, 4, 2, 8, {}, 1>(fv3::FusedArgs<float>)"
+    # two launches of two overlapping kernels each: spans 300 and 250
+    _write(d / "1_kernel_trace.csv", ["Kernel_Name", "Start_Timestamp", "End_Timestamp"],
+           [[k.format(0), 100, 400], [k.format(1), 150, 300], [k.format(0), 500, 700], [k.format(1), 510, 750],
+            ["void other(int)", 0, 1000]])
+    assert rs.spans(str(tmp_path), "k_fused_tp", 2) == [300.0, 250.0]
