@@ -188,7 +188,8 @@ int fv3_ctx_create(const fv3_bounds *bd, int npx, int npy, int npz, int ntiles, 
   if (hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c.join_ev[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c.join_ev[1], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c.join_ev[2], hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c.join_ev[2], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.prep_ev, hipEventDisableTiming) != hipSuccess) {
     fv3_ctx_destroy(h);
     return set_error("hipEventCreateWithFlags failed");
   }
@@ -261,6 +262,7 @@ int fv3_ctx_destroy(fv3_ctx *h) {
       (void)hipStreamDestroy(a);
     }
   if (c.fork_ev) (void)hipEventDestroy(c.fork_ev);
+  if (c.prep_ev) (void)hipEventDestroy(c.prep_ev);
   for (auto &e : c.join_ev)
     if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);

@@ -82,15 +82,17 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   // buffer), and the remap writes q_out.
   const bool fast_remap = std::is_same<T, float>::value && c.remap_mode == 0 && npz >= 8;
   T *q_tmp = nullptr, *pt_src = nullptr;
-  RemapCols cols{};
+  std::vector<RemapCols> cols(nt);  // column data of every tile
   if (fast_remap) {
+    if (remap_fast_check(npz, kord_tr, kord_tm, pt != nullptr)) return 1;
     const size_t b_q = ((sizeof(T) * qsize) + 255) & ~(size_t)255;
     const size_t b_pt = pt ? ((sizeof(T) * d.s2 * npz * nt) + 255) & ~(size_t)255 : 0;
-    char *sc = (char *)ctx_scratch(c, b_q + b_pt + remap_cols_bytes(npz, (long long)ni * nj));
+    const size_t b_col = remap_cols_bytes(npz, (long long)ni * nj);
+    char *sc = (char *)ctx_scratch(c, b_q + b_pt + b_col * nt);
     if (!sc) return 1;
     q_tmp = (T *)sc;
     pt_src = pt ? (T *)(sc + b_q) : nullptr;
-    cols = remap_cols_carve(sc + b_q + b_pt, npz, (long long)ni * nj);
+    for (int t = 0; t < nt; t++) cols[t] = remap_cols_carve(sc + b_q + b_pt + b_col * t, npz, (long long)ni * nj);
   } else if (cap > 1) {
     q_tmp = (T *)ctx_scratch(c, sizeof(T) * qsize);
     if (!q_tmp) return 1;
@@ -159,6 +161,25 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   }
   const bool timing = c.timing && !c.graph_safe;
   if (timing) FV3_HIP(hipEventRecord(c.ev[0], s));
+  // the remap's column prep (thicknesses only) and the copies it needs run on
+  // their own stream beside the transport; the field kernel joins it below
+  const bool early = fast_remap && c.concurrent_kinds && c.aux[3];
+  hipStream_t sp = early ? c.aux[3] : s;
+  auto remap_prep = [&]() -> int {
+    FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, sp));
+    if (pt) FV3_HIP(hipMemcpyAsync(pt_src, pt, sizeof(T) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, sp));
+    for (int t = 0; t < nt; t++)
+      if (remap_fast_prep_f32(c, npz, ak, bk, ptop, dp1 + t * d.s2 * npz, delp_out + t * d.s2 * npz,
+                              cols[t], sp))
+        return 1;
+    return 0;
+  };
+  if (early) {
+    FV3_HIP(hipEventRecord(c.fork_ev, s));
+    FV3_HIP(hipStreamWaitEvent(sp, c.fork_ev, 0));
+    if (remap_prep()) return 1;
+    FV3_HIP(hipEventRecord(c.prep_ev, sp));
+  }
   // each pass: the edge kinds fork onto the auxiliary streams and join back
   // before the next pass (which reads every face's previous pass)
   const bool conc = c.concurrent_kinds && c.aux[0] && c.aux[1] && c.aux[2];
@@ -187,17 +208,21 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   }
   if (timing) FV3_HIP(hipEventRecord(c.ev[1], s));
   // remap: source thickness is dp1, target lands in delp_out
-  FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, s));
-  if (fast_remap && pt)
-    FV3_HIP(hipMemcpyAsync(pt_src, pt, sizeof(T) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, s));
+  if (fast_remap) {
+    if (early)
+      FV3_HIP(hipStreamWaitEvent(s, c.prep_ev, 0));
+    else if (remap_prep())
+      return 1;
+  } else {
+    FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, s));
+  }
   if (timing) FV3_HIP(hipEventRecord(c.ev[2], s));
   if constexpr (std::is_same<T, float>::value) {
     if (fast_remap) {
-      for (int t = 0; t < nt; t++) {  // column data of one tile at a time
+      for (int t = 0; t < nt; t++) {
         const long long tq = a.tq, tl = d.s2 * npz;
-        if (remap_fast_f32(c, npz, nq, ak, bk, ptop, dp1 + t * tl, delp_out + t * tl, q_tmp + t * tq,
-                           q_out + t * tq, pt_src ? pt_src + t * tl : nullptr, pt ? pt + t * tl : nullptr,
-                           kord_tr, kord_tm, t_min, cols))
+        if (remap_fast_fields_f32(c, npz, nq, q_tmp + t * tq, q_out + t * tq, pt_src ? pt_src + t * tl : nullptr,
+                                  pt ? pt + t * tl : nullptr, kord_tr, kord_tm, t_min, cols[t], s))
           return 1;
       }
     }

@@ -82,10 +82,11 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // the fused transport's edge kinds run beside the lean kind on three
   // auxiliary streams (default priority), forked from and joined to the
-  // step's stream with events every pass
+  // step's stream with events every pass; a fourth runs the remap's column
+  // prep beside the transport
   bool concurrent_kinds = true;
-  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
-  hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
+  hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};  // [3]: the remap's column prep
+  hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr}, prep_ev = nullptr;
   int last_passes = 0;
   bool ev_valid = false;
 };

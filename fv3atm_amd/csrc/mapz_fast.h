@@ -28,13 +28,17 @@ struct RemapCols {
 size_t remap_cols_bytes(int km, long long ncol);
 RemapCols remap_cols_carve(void *base, int km, long long ncol);
 
-// Remap nq tracers (iv=0, kord_tr) from q_src into q_dst and, when pt_src is
-// given, pt (iv=1, kord_tm, t_min) into pt_dst, from source thickness dp1 to
-// the ak/bk levels at the source surface pressure; dp2 lands in delp_out.
-// Interior columns of one tile; sources and destinations must not overlap.
-int remap_fast_f32(Ctx &c, int km, int nq, const double *ak, const double *bk, double ptop,
-                   const double *dp1, double *delp_out, const float *q_src, float *q_dst,
-                   const float *pt_src, float *pt_dst, int kord_tr, int kord_tm, double t_min,
-                   const RemapCols &cols);
+// Remap of one tile's interior columns in two launches.  The prep (column
+// pressures, Thomas factors, target cut list; dp2 into delp_out) depends on
+// the thicknesses only, so it may run before or beside the transport; the
+// field kernel then remaps nq tracers (iv=0, kord_tr) from q_src into q_dst
+// and, when pt_src is given, pt (iv=1, kord_tm, t_min) into pt_dst.  Sources
+// and destinations must not overlap.
+int remap_fast_check(int km, int kord_tr, int kord_tm, bool with_pt);
+int remap_fast_prep_f32(Ctx &c, int km, const double *ak, const double *bk, double ptop,
+                        const double *dp1, double *delp_out, const RemapCols &cols, hipStream_t s);
+int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_dst, const float *pt_src,
+                          float *pt_dst, int kord_tr, int kord_tm, double t_min, const RemapCols &cols,
+                          hipStream_t s);
 
 }  // namespace fv3

@@ -447,25 +447,37 @@ RemapCols remap_cols_carve(void *base, int km, long long ncol) {
   return c;
 }
 
-int remap_fast_f32(Ctx &c, int km, int nq, const double *ak, const double *bk, double ptop,
-                   const double *dp1, double *delp_out, const float *q_src, float *q_dst,
-                   const float *pt_src, float *pt_dst, int kord_tr, int kord_tm, double t_min,
-                   const RemapCols &cols) {
-  const Dims &d = c.d;
+static int check_orders(int km, int kord_tr, int kord_tm, bool with_pt) {
   if (km < 8) return set_error("fast remap needs km >= 8 (got %d)", km);
   if (!(kord_tr > 7 && (kord_tr <= 10 || kord_tr > 16)))
     return set_error("kord_tr=%d: implemented orders are 8, 9, 10 and > 16", kord_tr);
-  if (pt_src && !(kord_tm > 7 && (kord_tm <= 10 || kord_tm > 16)))
+  if (with_pt && !(kord_tm > 7 && (kord_tm <= 10 || kord_tm > 16)))
     return set_error("kord_tm=%d: implemented orders are 8, 9, 10 and > 16", kord_tm);
+  return 0;
+}
+
+int remap_fast_prep_f32(Ctx &c, int km, const double *ak, const double *bk, double ptop,
+                        const double *dp1, double *delp_out, const RemapCols &cols, hipStream_t s) {
+  const Dims &d = c.d;
+  if (km < 8) return set_error("fast remap needs km >= 8 (got %d)", km);
   const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
   const long long ncol = (long long)ni * nj;
   PrepArgs pa{};
   pa.km = km; pa.ni = ni; pa.is = d.is; pa.js = d.js; pa.isd = d.isd; pa.jsd = d.jsd; pa.nid = d.nid;
   pa.ncol = ncol; pa.s2 = d.s2; pa.ptop = ptop; pa.dp1 = dp1; pa.ak = ak; pa.bk = bk;
   pa.delp_out = delp_out; pa.cols = cols;
-  hipLaunchKernelGGL(k_remap_prep, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, c.stream, pa);
+  hipLaunchKernelGGL(k_remap_prep, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s, pa);
   FV3_HIP(hipGetLastError());
+  return 0;
+}
 
+int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_dst, const float *pt_src,
+                          float *pt_dst, int kord_tr, int kord_tm, double t_min, const RemapCols &cols,
+                          hipStream_t s) {
+  const Dims &d = c.d;
+  if (check_orders(km, kord_tr, kord_tm, pt_src != nullptr)) return 1;
+  const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
+  const long long ncol = (long long)ni * nj;
   FieldArgs fa{};
   fa.km = km; fa.nb = (km + RF_B - 1) / RF_B; fa.ni = ni; fa.is = d.is; fa.js = d.js;
   fa.isd = d.isd; fa.jsd = d.jsd; fa.nid = d.nid; fa.nq = nq; fa.nf = nq + (pt_src ? 1 : 0);
@@ -478,10 +490,13 @@ int remap_fast_f32(Ctx &c, int km, int nq, const double *ak, const double *bk, d
   fa.nwg = (unsigned)nwg;
   const size_t lds = sizeof(float) * fa.nb * RF_COLS * RF_WAVES;
   if (lds > 64 * 1024) return set_error("fast remap supports km <= %d (got %d)", 64 * RF_B - 1, km);
-  hipLaunchKernelGGL((k_remap_fields<RF_B>), dim3((unsigned)nwg), dim3(RF_COLS * RF_WAVES), lds,
-                     c.stream, fa);
+  hipLaunchKernelGGL((k_remap_fields<RF_B>), dim3((unsigned)nwg), dim3(RF_COLS * RF_WAVES), lds, s, fa);
   FV3_HIP(hipGetLastError());
   return 0;
+}
+
+int remap_fast_check(int km, int kord_tr, int kord_tm, bool with_pt) {
+  return check_orders(km, kord_tr, kord_tm, with_pt);
 }
 
 }  // namespace fv3
