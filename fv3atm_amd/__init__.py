@@ -204,6 +204,13 @@ def step_timing(ctx: Context):
     return a.value, r.value, p.value
 
 
+def set_transport(ctx: Context, per_level: bool = True, q_split: int = 0):
+    """Sub-cycling rule of step_dev: per-level nsplit (tracer_2d_1L, the
+    default) or one nsplit for every level (tracer_2d), q_split if > 0."""
+    check(_ffi.lib().fv3_ctx_set_transport(ctx.handle, 1 if per_level else 0, q_split),
+          "fv3_ctx_set_transport")
+
+
 def set_concurrency(ctx: Context, on: bool = True):
     """Edge kernels of the fused transport beside the lean kernel on auxiliary
     streams (True, the default) or after it on the step's stream."""

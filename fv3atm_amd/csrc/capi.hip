@@ -707,6 +707,16 @@ int fv3_step_timing(fv3_ctx *h, double *transport_ms, double *remap_ms, int *pas
   return 0;
 }
 
+int fv3_ctx_set_transport(fv3_ctx *h, int per_level, int q_split) {
+  if (!h) return set_error("NULL context");
+  if (q_split < 0) return set_error("q_split must be >= 0 (got %d)", q_split);
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.step_per_level = per_level != 0;
+  c.step_q_split = q_split;
+  return 0;
+}
+
 int fv3_ctx_set_concurrency(fv3_ctx *h, int on) {
   if (!h) return set_error("NULL context");
   Ctx &c = h->c;

@@ -65,7 +65,8 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   if (!base) return 1;
   T *cmax = (T *)(base + o_cmax), *frac = (T *)(base + o_frac);
   int *nspl = (int *)(base + o_ns), *flag = c.graph_safe ? c.step_flag : (int *)(base + o_flag);
-  if (device_nsplit<T>(c, cx, cy, true, 0, cmax, nspl, frac)) return 1;
+  if (device_nsplit<T>(c, cx, cy, c.step_per_level, c.step_per_level ? 0 : c.step_q_split, cmax, nspl, frac))
+    return 1;
   int cap = nsplit_cap;
   if (cap <= 0) {
     std::vector<int> h(npz);

@@ -75,6 +75,10 @@ struct Ctx {
   int fused_cfg = 0, fused_h = 0;
   // FP32 step remap: 0 column-shared fast kernel, 1 one thread per column
   int remap_mode = 0;
+  // fused step's sub-cycling: per-level nsplit(k) (tracer_2d_1L) or one
+  // nsplit for every level (tracer_2d), forced to q_split when > 0
+  bool step_per_level = true;
+  int step_q_split = 0;
   // graph-safe steps: no host synchronisation, no memset, no allocation; the
   // nsplit_cap check lands in step_flag for fv3_step_status
   bool graph_safe = false;

@@ -137,6 +137,12 @@ module fv3atm_amd_cabi
        integer(c_int), value :: on
      end function fv3_ctx_set_concurrency
 
+     integer(c_int) function fv3_ctx_set_transport(ctx, per_level, q_split) bind(C, name="fv3_ctx_set_transport")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: per_level, q_split
+     end function fv3_ctx_set_transport
+
      integer(c_int) function fv3_step_timing(ctx, transport_ms, remap_ms, passes) bind(C, name="fv3_step_timing")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx
@@ -154,7 +160,8 @@ module fv3atm_amd_cabi
   public :: fv3_last_error, fv3_ctx_create, fv3_ctx_destroy, fv3_ctx_set_grid, fv3_fv_tp_2d
   public :: fv3_tracer_2d_1l, fv3_tracer_2d, fv3_map1_ppm, fv3_map_scalar, fv3_mapn_tracer
   public :: fv3_cs_profile, fv3_lagrangian_to_eulerian_tracers, fv3_step_dev
-  public :: fv3_ctx_set_timing, fv3_ctx_set_launch, fv3_ctx_set_concurrency, fv3_step_timing
+  public :: fv3_ctx_set_timing, fv3_ctx_set_launch, fv3_ctx_set_concurrency, fv3_ctx_set_transport
+  public :: fv3_step_timing
 
   ! one cached context per (bounds, npx, npy, npz); the metrics are uploaded
   ! when it is created

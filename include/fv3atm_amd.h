@@ -148,6 +148,14 @@ int fv3_ctx_set_launch(fv3_ctx *ctx, int cfg, int rows, int remap);
  * joined to the step's stream every pass (on = 1, the default), or after it
  * on the step's stream (on = 0).  Results are identical either way. */
 int fv3_ctx_set_concurrency(fv3_ctx *ctx, int on);
+/* Sub-cycling rule of fv3_step_dev (no counterpart upstream; upstream the
+ * choice is which routine fv_dynamics calls): per_level = 1 (default) is
+ * fv_tracer2d.F90 tracer_2d_1L, nsplit(k) = int(1 + cmax(k)) per level;
+ * per_level = 0 is fv_tracer2d.F90 tracer_2d, one nsplit =
+ * int(1 + max_k cmax(k)) for every level, or q_split when q_split > 0, with
+ * trdm = 0 and id_divg = 0 (the step has no deln_flux damping and does not
+ * overwrite dp1 with the divergence). */
+int fv3_ctx_set_transport(fv3_ctx *ctx, int per_level, int q_split);
 int fv3_step_timing(fv3_ctx *ctx, double *transport_ms, double *remap_ms, int *passes);
 /* Graph-safe steps (no counterpart upstream): with on != 0, fv3_step_dev
  * makes no host synchronisation, allocation or memset once its scratch is

@@ -23,12 +23,13 @@ def _dev(a, dtype):
         device="cuda", dtype=dtype)
 
 
-def run_step(case, form, nsplit_cap=0, rows=0, cfg=0, concurrent=True):
+def run_step(case, form, nsplit_cap=0, rows=0, cfg=0, concurrent=True, per_level=True, q_split=0):
     dt = torch.float32 if form == fa.FORM_FUSED32 else torch.float64
     ctx = fa.Context(case.grids, case.npz)
     if rows or cfg:
         fa.set_launch(ctx, cfg, rows, 0)
     fa.set_concurrency(ctx, concurrent)
+    fa.set_transport(ctx, per_level, q_split)
     q_in = _dev(case.q, dt)
     q_out = torch.zeros_like(q_in)
     pt = _dev(case.pt, dt) if case.pt is not None else None
@@ -182,3 +183,22 @@ def test_step_concurrent_kinds_equal_serial(gpu_ready, form):
         print(case.name, "max |concurrent - serial|", float(np.abs(a["q"] - b["q"]).max()), flush=True)
         assert np.array_equal(interior(a["q"]), interior(b["q"]))
         assert np.array_equal(interior(a["pt"]), interior(b["pt"]))
+
+
+@pytest.mark.parametrize("q_split", [0, 3])
+@pytest.mark.parametrize("name", ["C100L6Q7jet", "6xC32L8Q2"])
+def test_step_tracer_2d_rule(gpu_ready, name, q_split):
+    # tracer_2d sub-cycling (one nsplit for every level; forced to q_split when
+    # given): FP64 to the oracle's tracer_2d bit for bit, FP32 to 1e-5
+    case = CASES[name]()
+    case.q_split = q_split
+    adv = oracle.tracer_2d(case, "2d")
+    q, delp, pt = oracle.remap_tracers(case, adv["q"], case.dp1, case.pt)
+    print(name, "q_split", q_split, "nsplit", np.unique(adv["nsplit"]).tolist(), flush=True)
+    got = run_step(case, fa.FORM_FUSED64, per_level=False, q_split=q_split)
+    assert_close(interior(got["q"]), interior(q), 1e-12, f"{name} tracer_2d q", floor=1e-300)
+    assert_close(interior(got["pt"]), interior(pt), 1e-12, f"{name} tracer_2d pt")
+    got32 = run_step(case, fa.FORM_FUSED32, per_level=False, q_split=q_split)
+    for iq in range(case.nq):
+        e = rel_l2(interior(got32["q"][:, :, :, iq]), interior(q[:, :, :, iq]))
+        assert e <= 1e-5, (iq, e)
