@@ -210,10 +210,19 @@ def run_sphere(args, torch, fa, cases):
     dm32 = ((m32 - m0).abs() / mabs).max().item()
     del q32, q64
     log(f"{passes} passes; FP32 vs FP64 rel L2 max {rel:.2e}; global mass drift FP64 {dm64:.2e}, FP32 {dm32:.2e}")
-    ms, t_adv, t_rem = run_form(fa, ctx, fa.FORM_FUSED32, d, case, args.steps, args.warmup, passes, torch,
-                                timing=True)
+    extra = {}
+    if args.graph:  # the step captured once per buffer direction, replayed
+        ms, bad = run_graph(fa, ctx, d, case, args.steps, args.warmup, passes, torch)
+        if bad:
+            raise SystemExit(f"graph replays: nsplit above the captured {passes} passes")
+        extra = {"graph": True}
+        log(f"fused FP32 sphere, hipGraph replays: {ms:.2f} ms/step")
+    else:
+        ms, t_adv, t_rem = run_form(fa, ctx, fa.FORM_FUSED32, d, case, args.steps, args.warmup, passes, torch,
+                                    timing=True)
+        extra = {"transport_ms": round(float(np.mean(t_adv)), 3), "remap_ms": round(float(np.mean(t_rem)), 3)}
+        log(f"fused FP32 sphere: {ms:.2f} ms/step (transport {np.mean(t_adv):.2f}, remap {np.mean(t_rem):.2f})")
     value = T * N * N * L * Q / (ms * 1e-3) / 1e9
-    log(f"fused FP32 sphere: {ms:.2f} ms/step (transport {np.mean(t_adv):.2f}, remap {np.mean(t_rem):.2f})")
     print(json.dumps({
         "metric": f"Gcell-tracer updates/s 6xC{N}L{L}x{Q} tracers, fused FP32 step", "value": round(value, 3),
         "unit": "Gcell-tracer updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -222,9 +231,77 @@ def run_sphere(args, torch, fa, cases):
         "config": {"workload": f"6 x C{N} L{L} x {Q} tracers, solid-body rotation over the cube corners: "
                                f"tracer_2d_1L hord 8 + kord 9 tracer remap, fused FP32 form",
                    "n": N, "npz": L, "nq": Q, "ntiles": T, "passes": passes},
-        "transport_ms": round(float(np.mean(t_adv)), 3), "remap_ms": round(float(np.mean(t_rem)), 3),
-        "fp32_rel_l2_max": rel, "mass_drift_fp64": dm64, "mass_drift_fp32": dm32}), flush=True)
+        **extra, "fp32_rel_l2_max": rel, "mass_drift_fp64": dm64, "mass_drift_fp32": dm32}), flush=True)
     ctx.close()
+
+
+def run_mixed_precision(args, torch, fa, cases):
+    """One C384 tile, 64 levels, 10 tracers, deformational winds, forced
+    nsplit = 2 (tracer_2d rule, q_split 2), kord 9 remap: the fused FP32 step
+    against the fused FP64 step and the FP64 CPU oracle, per tracer (relative
+    L2 and Linf, mass change), and the FP32 and FP64 step times.  Prints one
+    JSON line."""
+    from oracle import oracle
+    case = cases.config(1)
+    N, L, Q = case.n, case.npz, case.nq
+    nid = N + 6
+    log(f"case C{N} L{L} Q{Q} q_split {case.q_split}")
+    area = torch.from_numpy(np.ascontiguousarray(case.grids[0].area.T)).cuda()[3:-3, 3:-3]
+
+    def run(form, dtype):
+        ctx = fa.Context(case.grids, L)
+        fa.set_transport(ctx, False, case.q_split)
+        fa.set_timing(ctx, True)
+        d = device_inputs(case, torch, dtype)
+        q0 = d["q_in"].clone()
+        fa.step_dev(ctx, form, Q, d["q_in"], d["q_out"], None, d["dp1"], d["delp_out"], d["cx"], d["cy"],
+                    d["mfx"], d["mfy"], d["ak"], d["bk"], case.ptop, case.hord, case.kord_tr, 10, 184.0, 0)
+        torch.cuda.synchronize()
+        out = d["q_out"].view(Q, L, nid, nid)[:, :, 3:-3, 3:-3].double().clone()
+        dpo = d["delp_out"].view(L, nid, nid)[:, 3:-3, 3:-3].clone()
+        ms, _, _ = run_form(fa, ctx, form, d, case, args.steps, args.warmup, case.q_split, torch)
+        ctx.close()
+        return out, dpo, q0.view(Q, L, nid, nid)[:, :, 3:-3, 3:-3].double(), ms
+
+    q64, dp64, q0, ms64 = run(fa.FORM_FUSED64, torch.float64)
+    q32, dp32, _, ms32 = run(fa.FORM_FUSED32, torch.float32)
+    t0 = time.perf_counter()
+    adv = oracle.tracer_2d(case, "2d")
+    qo, dpo_, _ = oracle.remap_tracers(case, adv["q"], case.dp1, None)
+    log(f"CPU oracle step in {time.perf_counter() - t0:.1f} s")
+    qo = torch.from_numpy(np.ascontiguousarray(np.transpose(qo[3:-3, 3:-3], (3, 2, 1, 0)))).cuda()
+    dp1 = torch.from_numpy(np.ascontiguousarray(case.dp1.T)).cuda()[:, 3:-3, 3:-3]
+    m0 = (q0 * dp1 * area).sum(dim=(1, 2, 3))
+    mabs = (q0.abs() * dp1 * area).sum(dim=(1, 2, 3))
+
+    def errs(q, ref):
+        l2 = (torch.linalg.vector_norm((q - ref).reshape(Q, -1), dim=1) /
+              torch.linalg.vector_norm(ref.reshape(Q, -1), dim=1))
+        li = (q - ref).abs().amax(dim=(1, 2, 3)) / ref.abs().amax(dim=(1, 2, 3))
+        return [float(x) for x in l2], [float(x) for x in li]
+
+    l2_64, li_64 = errs(q64, qo)
+    l2_32, li_32 = errs(q32, qo)
+    # one open tile: its mass changes by what crosses the tile edges, so the
+    # FP32 form's conservation error is its mass against the FP64 form's
+    m32 = (q32 * dp32 * area).sum(dim=(1, 2, 3))
+    m64 = (q64 * dp64 * area).sum(dim=(1, 2, 3))
+    drift32 = [float(x) for x in (m32 - m64).abs() / mabs]
+    flux64 = [float(x) for x in (m64 - m0) / mabs]
+    for iq in range(Q):
+        log(f"tracer {iq}: FP32 rel L2 {l2_32[iq]:.2e} Linf {li_32[iq]:.2e} mass vs FP64 {drift32[iq]:.1e}; "
+            f"FP64 rel L2 {l2_64[iq]:.1e}")
+    cells = N * N * L * Q
+    print(json.dumps({
+        "metric": f"Gcell-tracer updates/s C{N}L{L}x{Q} tracers, fused FP32 vs FP64 error report",
+        "value": round(cells / (ms32 * 1e-3) / 1e9, 3), "unit": "Gcell-tracer updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms32, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"C{N} tile L{L} x {Q} tracers, deformational winds, tracer_2d with "
+                               f"q_split {case.q_split}, kord 9 tracer remap", "n": N, "npz": L, "nq": Q},
+        "fp64_ms_per_step": round(ms64, 3), "fp32_rel_l2_vs_oracle": l2_32, "fp32_linf_vs_oracle": li_32,
+        "fp64_rel_l2_vs_oracle": l2_64, "fp32_mass_vs_fp64": drift32, "fp64_tile_edge_mass_flux": flux64}),
+        flush=True)
 
 
 def main():
@@ -251,6 +328,8 @@ def main():
                     help="time the step as hipGraph replays (graph-safe mode)")
     ap.add_argument("--sphere", action="store_true",
                     help="the six-face C384 L127 Q20 sphere instead of the C768 tile")
+    ap.add_argument("--mixed", action="store_true",
+                    help="the C384 L64 Q10 FP32-against-FP64 error report instead of the C768 tile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -265,6 +344,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.sphere:
         run_sphere(args, torch, fa, cases)
+        return
+    if args.mixed:
+        run_mixed_precision(args, torch, fa, cases)
         return
     if args.calib:  # reads 1 GiB and writes 1 GiB in one elementwise kernel
         x = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
