@@ -328,6 +328,9 @@ def main():
                     help="time the step as hipGraph replays (graph-safe mode)")
     ap.add_argument("--sphere", action="store_true",
                     help="the six-face C384 L127 Q20 sphere instead of the C768 tile")
+    ap.add_argument("--pipeline", type=int, default=None,
+                    help="tracers per transport chunk, each chunk remapped beside the next chunk's "
+                         "transport (-1: one round of the launch shape; 0: off, the library's default)")
     ap.add_argument("--mixed", action="store_true",
                     help="the C384 L64 Q10 FP32-against-FP64 error report instead of the C768 tile")
     args = ap.parse_args()
@@ -443,6 +446,8 @@ def main():
         launch, rows = best[1], best[2]
     fa.set_launch(ctx, launch, rows, args.remap)
     fa.set_concurrency(ctx, not args.serial_kinds)
+    if args.pipeline is not None:
+        fa.set_pipeline(ctx, args.pipeline)
     out["launch_cfg"] = launch
     out["concurrent_kinds"] = not args.serial_kinds
     out["remap_kernel"] = args.remap

@@ -217,6 +217,13 @@ def set_concurrency(ctx: Context, on: bool = True):
     check(_ffi.lib().fv3_ctx_set_concurrency(ctx.handle, 1 if on else 0), "fv3_ctx_set_concurrency")
 
 
+def set_pipeline(ctx: Context, tracers_per_chunk: int = 0):
+    """FP32 step: transport the tracers in chunks (-1: one round of the launch
+    shape; 0, the default: all at once) and remap each chunk beside the next
+    chunk's transport.  Results do not depend on the chunking."""
+    check(_ffi.lib().fv3_ctx_set_pipeline(ctx.handle, int(tracers_per_chunk)), "fv3_ctx_set_pipeline")
+
+
 def set_launch(ctx: Context, cfg: int = 0, rows: int = 0, remap: int = 0):
     """Kernel choices of the FP32 step: transport launch shape (cfg, rows;
     results unchanged) and remap kernel (0 column-shared, 1 per column)."""

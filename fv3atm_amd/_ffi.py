@@ -50,6 +50,7 @@ SIGNATURES = {
     "fv3_step_timing": (I, [P, DP, DP, IP]),
     "fv3_ctx_set_launch": (I, [P, I, I, I]),
     "fv3_ctx_set_concurrency": (I, [P, I]),
+    "fv3_ctx_set_pipeline": (I, [P, I]),
     "fv3_ctx_set_transport": (I, [P, I, I]),
     "fv3_ctx_set_graph_safe": (I, [P, I]),
     "fv3_step_status": (I, [P, IP]),

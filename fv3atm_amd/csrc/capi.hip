@@ -189,7 +189,8 @@ int fv3_ctx_create(const fv3_bounds *bd, int npx, int npy, int npz, int ntiles, 
       hipEventCreateWithFlags(&c.join_ev[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c.join_ev[1], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c.join_ev[2], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c.prep_ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c.prep_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.chunk_ev, hipEventDisableTiming) != hipSuccess) {
     fv3_ctx_destroy(h);
     return set_error("hipEventCreateWithFlags failed");
   }
@@ -263,6 +264,7 @@ int fv3_ctx_destroy(fv3_ctx *h) {
     }
   if (c.fork_ev) (void)hipEventDestroy(c.fork_ev);
   if (c.prep_ev) (void)hipEventDestroy(c.prep_ev);
+  if (c.chunk_ev) (void)hipEventDestroy(c.chunk_ev);
   for (auto &e : c.join_ev)
     if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -722,6 +724,16 @@ int fv3_ctx_set_concurrency(fv3_ctx *h, int on) {
   Ctx &c = h->c;
   std::lock_guard<std::mutex> lk(c.mu);
   c.concurrent_kinds = on != 0;
+  return 0;
+}
+
+int fv3_ctx_set_pipeline(fv3_ctx *h, int tracers_per_chunk) {
+  if (!h) return set_error("NULL context");
+  if (tracers_per_chunk < -1)
+    return set_error("tracers per chunk must be -1 (one round), 0 (off) or positive (got %d)", tracers_per_chunk);
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.pipe_chunk = tracers_per_chunk;
   return 0;
 }
 

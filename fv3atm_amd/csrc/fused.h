@@ -16,6 +16,8 @@ struct KindStreams {
 };
 int launch_fused_f64(const FusedArgs<double> &a, const KindStreams &s);
 int launch_fused_f32(const FusedArgs<float> &a, int cfg, const KindStreams &s);
+// tracers per round (waves x tracers per wave) of the lean kind of cfg
+int fused32_round(int cfg);
 
 // per-level sub-cycle counts on the device (cmax, nsplit; no scaling)
 template <typename T>

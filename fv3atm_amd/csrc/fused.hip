@@ -181,35 +181,79 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     if (remap_prep()) return 1;
     FV3_HIP(hipEventRecord(c.prep_ev, sp));
   }
-  // each pass: the edge kinds fork onto the auxiliary streams and join back
-  // before the next pass (which reads every face's previous pass)
-  const bool conc = c.concurrent_kinds && c.aux[0] && c.aux[1] && c.aux[2];
-  const KindStreams ks{{s, conc ? c.aux[0] : s, conc ? c.aux[1] : s, conc ? c.aux[2] : s}};
-  for (int it = 1; it <= cap; it++) {
-    a.it = it;
-    if (conc) {
-      FV3_HIP(hipEventRecord(c.fork_ev, s));
-      for (int i = 0; i < 3; i++) FV3_HIP(hipStreamWaitEvent(c.aux[i], c.fork_ev, 0));
+  // FP32 pipelining: the tracers go through the passes in chunks of cq, and
+  // each chunk's remap runs on aux[3] (behind the column prep) while the
+  // next chunk is transported; pt is remapped with the last chunk
+  int cq = nq;
+  if constexpr (std::is_same<T, float>::value) {
+    if (early && c.pipe_chunk != 0) {
+      cq = c.pipe_chunk > 0 ? c.pipe_chunk : fused32_round(c.fused_cfg);
+      cq = std::max(1, std::min(cq, nq));
     }
-    int rc;
-    if constexpr (std::is_same<T, double>::value)
-      rc = launch_fused_f64(a, ks);
-    else
-      rc = launch_fused_f32(a, c.fused_cfg, ks);
-    if (rc) return rc;
-    if (conc)
-      for (int i = 0; i < 3; i++) {
+  }
+  const bool pipe = cq < nq;
+  // each pass: the edge kinds fork onto auxiliary streams and join back
+  // before the next pass (which reads every face's previous pass); when
+  // pipelined they share two streams, so that with the remap's stream the
+  // step uses four
+  const bool conc = c.concurrent_kinds && c.aux[0] && c.aux[1] && c.aux[2];
+  const KindStreams ks = !conc ? KindStreams{{s, s, s, s}}
+                         : (pipe ? KindStreams{{s, c.aux[0], c.aux[0], c.aux[1]}}
+                                 : KindStreams{{s, c.aux[0], c.aux[1], c.aux[2]}});
+  const int nfork = !conc ? 0 : (pipe ? 2 : 3);
+  const T *const pass_in = a.q_in;
+  T *const pass_out = a.q_out, *const pass_tmp = a.q_tmp;
+  const long long tq = a.tq, tl = d.s2 * npz;
+  for (int q0 = 0; q0 < nq; q0 += cq) {
+    const int nqc = std::min(cq, nq - q0);
+    const long long qoff = (long long)q0 * npz * d.s2;
+    a.nq = nqc;
+    a.q_in = pass_in + qoff;
+    a.q_out = pass_out + qoff;
+    a.q_tmp = pass_tmp ? pass_tmp + qoff : nullptr;
+    for (int it = 1; it <= cap; it++) {
+      a.it = it;
+      if (nfork) {
+        FV3_HIP(hipEventRecord(c.fork_ev, s));
+        for (int i = 0; i < nfork; i++) FV3_HIP(hipStreamWaitEvent(c.aux[i], c.fork_ev, 0));
+      }
+      int rc;
+      if constexpr (std::is_same<T, double>::value)
+        rc = launch_fused_f64(a, ks);
+      else
+        rc = launch_fused_f32(a, c.fused_cfg, ks);
+      if (rc) return rc;
+      for (int i = 0; i < nfork; i++) {
         FV3_HIP(hipEventRecord(c.join_ev[i], c.aux[i]));
         FV3_HIP(hipStreamWaitEvent(s, c.join_ev[i], 0));
       }
+    }
+    if constexpr (std::is_same<T, float>::value) {
+      if (pipe) {
+        FV3_HIP(hipEventRecord(c.chunk_ev, s));
+        FV3_HIP(hipStreamWaitEvent(sp, c.chunk_ev, 0));
+        const bool last = q0 + nqc >= nq;
+        for (int t = 0; t < nt; t++)
+          if (remap_fast_fields_f32(c, npz, nqc, q_tmp + t * tq + qoff, q_out + t * tq + qoff,
+                                    last && pt_src ? pt_src + t * tl : nullptr, last && pt ? pt + t * tl : nullptr,
+                                    kord_tr, kord_tm, t_min, cols[t], sp))
+            return 1;
+      }
+    }
   }
+  a.nq = nq;
   if (nsplit_cap > 0) {
     hipLaunchKernelGGL(k_nsplit_check, dim3(1), dim3(1), 0, s, nspl, npz, cap, flag);
     FV3_HIP(hipGetLastError());
   }
   if (timing) FV3_HIP(hipEventRecord(c.ev[1], s));
-  // remap: source thickness is dp1, target lands in delp_out
-  if (fast_remap) {
+  // remap: source thickness is dp1, target lands in delp_out; pipelined, the
+  // remap phase is the wait for the last chunks' remaps
+  if (pipe) {
+    if (timing) FV3_HIP(hipEventRecord(c.ev[2], s));
+    FV3_HIP(hipEventRecord(c.prep_ev, sp));
+    FV3_HIP(hipStreamWaitEvent(s, c.prep_ev, 0));
+  } else if (fast_remap) {
     if (early)
       FV3_HIP(hipStreamWaitEvent(s, c.prep_ev, 0));
     else if (remap_prep())
@@ -217,11 +261,10 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   } else {
     FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, s));
   }
-  if (timing) FV3_HIP(hipEventRecord(c.ev[2], s));
+  if (timing && !pipe) FV3_HIP(hipEventRecord(c.ev[2], s));
   if constexpr (std::is_same<T, float>::value) {
-    if (fast_remap) {
+    if (fast_remap && !pipe) {
       for (int t = 0; t < nt; t++) {
-        const long long tq = a.tq, tl = d.s2 * npz;
         if (remap_fast_fields_f32(c, npz, nq, q_tmp + t * tq, q_out + t * tq, pt_src ? pt_src + t * tl : nullptr,
                                   pt ? pt + t * tl : nullptr, kord_tr, kord_tm, t_min, cols[t], s))
           return 1;

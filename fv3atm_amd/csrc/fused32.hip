@@ -33,6 +33,12 @@ static int go(const FusedArgs<float> &a, const KindStreams &s) {
   return 0;
 }
 
+// W x NT of the lean kind of each case below
+static const int kRound[] = {8, 8, 4, 4, 6, 2, 1, 2, 4, 8, 4, 8, 16, 12, 8};
+static_assert(sizeof(kRound) / sizeof(kRound[0]) == FUSED32_NCFG, "one entry per launch configuration");
+
+int fused32_round(int cfg) { return cfg >= 0 && cfg < FUSED32_NCFG ? kRound[cfg] : 8; }
+
 int launch_fused_f32(const FusedArgs<float> &a, int cfg, const KindStreams &s) {
   switch (cfg) {
     case 0: return go<4, 2>(a, s);

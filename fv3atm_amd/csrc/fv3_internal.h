@@ -91,6 +91,11 @@ struct Ctx {
   bool concurrent_kinds = true;
   hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};  // [3]: the remap's column prep
   hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr}, prep_ev = nullptr;
+  // FP32 step pipelining: the tracers are transported in chunks of this many
+  // (-1: one round of the launch shape, 0: all at once, the default), and
+  // each chunk's remap runs on aux[3] beside the next chunk's transport
+  int pipe_chunk = 0;
+  hipEvent_t chunk_ev = nullptr;
   int last_passes = 0;
   bool ev_valid = false;
 };

@@ -137,6 +137,12 @@ module fv3atm_amd_cabi
        integer(c_int), value :: on
      end function fv3_ctx_set_concurrency
 
+     integer(c_int) function fv3_ctx_set_pipeline(ctx, tracers_per_chunk) bind(C, name="fv3_ctx_set_pipeline")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: tracers_per_chunk
+     end function fv3_ctx_set_pipeline
+
      integer(c_int) function fv3_ctx_set_transport(ctx, per_level, q_split) bind(C, name="fv3_ctx_set_transport")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
@@ -160,7 +166,8 @@ module fv3atm_amd_cabi
   public :: fv3_last_error, fv3_ctx_create, fv3_ctx_destroy, fv3_ctx_set_grid, fv3_fv_tp_2d
   public :: fv3_tracer_2d_1l, fv3_tracer_2d, fv3_map1_ppm, fv3_map_scalar, fv3_mapn_tracer
   public :: fv3_cs_profile, fv3_lagrangian_to_eulerian_tracers, fv3_step_dev
-  public :: fv3_ctx_set_timing, fv3_ctx_set_launch, fv3_ctx_set_concurrency, fv3_ctx_set_transport
+  public :: fv3_ctx_set_timing, fv3_ctx_set_launch, fv3_ctx_set_concurrency, fv3_ctx_set_transport, &
+            fv3_ctx_set_pipeline
   public :: fv3_step_timing
 
   ! one cached context per (bounds, npx, npy, npz); the metrics are uploaded

@@ -148,6 +148,12 @@ int fv3_ctx_set_launch(fv3_ctx *ctx, int cfg, int rows, int remap);
  * joined to the step's stream every pass (on = 1, the default), or after it
  * on the step's stream (on = 0).  Results are identical either way. */
 int fv3_ctx_set_concurrency(fv3_ctx *ctx, int on);
+/* No counterpart upstream: the FP32 step transports the tracers in chunks
+ * of tracers_per_chunk (-1: one round of the launch shape, 8 tracers for the
+ * default shape; 0, the default: all tracers at once) and remaps each chunk
+ * on an auxiliary stream while the next chunk is transported (with
+ * concurrency on).  The per-tracer results do not depend on the chunking. */
+int fv3_ctx_set_pipeline(fv3_ctx *ctx, int tracers_per_chunk);
 /* Sub-cycling rule of fv3_step_dev (no counterpart upstream; upstream the
  * choice is which routine fv_dynamics calls): per_level = 1 (default) is
  * fv_tracer2d.F90 tracer_2d_1L, nsplit(k) = int(1 + cmax(k)) per level;

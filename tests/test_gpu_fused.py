@@ -23,12 +23,15 @@ def _dev(a, dtype):
         device="cuda", dtype=dtype)
 
 
-def run_step(case, form, nsplit_cap=0, rows=0, cfg=0, concurrent=True, per_level=True, q_split=0):
+def run_step(case, form, nsplit_cap=0, rows=0, cfg=0, concurrent=True, per_level=True, q_split=0,
+             pipeline=None):
     dt = torch.float32 if form == fa.FORM_FUSED32 else torch.float64
     ctx = fa.Context(case.grids, case.npz)
     if rows or cfg:
         fa.set_launch(ctx, cfg, rows, 0)
     fa.set_concurrency(ctx, concurrent)
+    if pipeline is not None:
+        fa.set_pipeline(ctx, pipeline)
     fa.set_transport(ctx, per_level, q_split)
     q_in = _dev(case.q, dt)
     q_out = torch.zeros_like(q_in)
@@ -183,6 +186,25 @@ def test_step_concurrent_kinds_equal_serial(gpu_ready, form):
         print(case.name, "max |concurrent - serial|", float(np.abs(a["q"] - b["q"]).max()), flush=True)
         assert np.array_equal(interior(a["q"]), interior(b["q"]))
         assert np.array_equal(interior(a["pt"]), interior(b["pt"]))
+
+
+@pytest.mark.parametrize("chunk", [1, 3, -1])
+def test_step_pipelined_chunks_equal_whole(gpu_ready, chunk):
+    # tracers transported in chunks, each chunk remapped beside the next
+    # chunk's transport: the same bits as all tracers at once (one tile and
+    # six tiles; pt goes with the last chunk)
+    for mk in (lambda: cases.make_case("C100L12Q7jet", n=100, npz=12, nq=7, jet=True, courant=1.9,
+                                       delp_kind=1, delp_amp=(0.05, 0.15), seed=37, with_pt=True),
+               lambda: cases.make_case("6xC48L8Q5", n=48, npz=8, nq=5, ntiles=6, courant=1.7,
+                                       delp_kind=1, delp_amp=(0.05, 0.15), seed=41, with_pt=True)):
+        case = mk()
+        a = run_step(case, fa.FORM_FUSED32, pipeline=chunk, cfg=2 if chunk == -1 else 0)
+        b = run_step(case, fa.FORM_FUSED32, pipeline=0, cfg=2 if chunk == -1 else 0)
+        print(case.name, "chunk", chunk, "max |pipelined - whole|", float(np.abs(a["q"] - b["q"]).max()),
+              flush=True)
+        assert np.array_equal(interior(a["q"]), interior(b["q"]))
+        assert np.array_equal(interior(a["pt"]), interior(b["pt"]))
+        assert np.array_equal(interior(a["delp"]), interior(b["delp"]))
 
 
 @pytest.mark.parametrize("q_split", [0, 3])
