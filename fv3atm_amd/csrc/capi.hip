@@ -306,7 +306,7 @@ int fv3_ctx_set_grid(fv3_ctx *h, int tile, const fv3_grid *g) {
   {  // FP32 combined upwind factors of this tile for the fused FP32 form
     const int nid = c.d.nid, njd = c.d.njd;
     const long long nx = (long long)(nid + 1) * njd, ny = (long long)nid * (njd + 1);
-    std::vector<float> f(2 * nx + 2 * ny, 0.0f);
+    std::vector<float> f(2 * nx + 2 * ny + s2, 0.0f);
     for (int j = 0; j < njd; j++)
       for (int i = 1; i < nid; i++) {  // x-face i between cells i-1 and i (0-based from isd)
         const long long fo = (long long)j * (nid + 1) + i;
@@ -321,6 +321,22 @@ int fv3_ctx_set_grid(fv3_ctx *h, int tile, const fv3_grid *g) {
         f[2 * nx + fo] = (float)(g->dya[(long long)(j - 1) * nid + i] * dxv * g->sin_sg[3 * s2 + (long long)(j - 1) * nid + i]);
         f[2 * nx + ny + fo] = (float)(g->dya[(long long)j * nid + i] * dxv * g->sin_sg[1 * s2 + (long long)j * nid + i]);
       }
+    // the cube-edge face value of xppm between cells e-1 and e (e = is at the
+    // left edge, ie + 1 at the right one) as a weighted sum of cells e-2..e+1:
+    // 1/2 [((2 d(e-1) + d(e-2)) q(e-1) - d(e-1) q(e-2)) / (d(e-2) + d(e-1))
+    //    + ((2 d(e) + d(e+1)) q(e) - d(e) q(e+1)) / (d(e) + d(e+1))], d = dxa
+    if (g->bounded_domain == 0 && g->grid_type < 3)
+      for (int j = 0; j < njd; j++)
+        for (int e : {c.d.is - c.d.isd, c.d.ie + 1 - c.d.isd}) {
+          if (e < 2 || e + 1 >= nid) continue;
+          const double *dr = g->dxa + (long long)j * nid;
+          const double da = dr[e - 2], db = dr[e - 1], dc = dr[e], de = dr[e + 1];
+          float *w = f.data() + 2 * nx + 2 * ny + (long long)j * nid;
+          w[e - 2] = (float)(-0.5 * db / (da + db));
+          w[e - 1] = (float)(0.5 * (2.0 * db + da) / (da + db));
+          w[e] = (float)(0.5 * (2.0 * dc + de) / (dc + de));
+          w[e + 1] = (float)(-0.5 * dc / (dc + de));
+        }
     if (!c.gfac32) FV3_HIP(hipMalloc(&c.gfac32, sizeof(float) * f.size() * c.ntiles));
     FV3_HIP(hipMemcpy(c.gfac32 + (long long)t * f.size(), f.data(), sizeof(float) * f.size(),
                       hipMemcpyHostToDevice));

@@ -158,7 +158,8 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     a.gxm = a.gxp + (long long)(d.nid + 1) * d.njd;
     a.gyp = a.gxm + (long long)(d.nid + 1) * d.njd;
     a.gym = a.gyp + (long long)d.nid * (d.njd + 1);
-    a.tgf = 2 * (long long)(d.nid + 1) * d.njd + 2 * (long long)d.nid * (d.njd + 1);
+    a.gex = a.gym + (long long)d.nid * (d.njd + 1);
+    a.tgf = 2 * (long long)(d.nid + 1) * d.njd + 2 * (long long)d.nid * (d.njd + 1) + (long long)d.nid * d.njd;
   }
   const bool timing = c.timing && !c.graph_safe;
   if (timing) FV3_HIP(hipEventRecord(c.ev[0], s));

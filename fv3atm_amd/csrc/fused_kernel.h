@@ -23,6 +23,7 @@ struct StripCtx {
   const T *cxk, *cyk, *mfxk, *mfyk;
   const double *dp1k;
   long long tg, tdx, tdy, tgf;  // this tile's offsets in the metric arrays (s2, dx, dy, gfac)
+  EdgeCoef ec;                  // FP32 edge strips: the lane's cube-edge coefficients
 };
 
 // per tracer streaming state
@@ -239,10 +240,12 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   const T mfys = in.mfy * sc.frac;
   const T mfxs = in.mfx * sc.frac;
   const T mfxp = from_right(mfxs);
+  // edge strips: dxa of the rows (FP64), or the cells' edge face weights (FP32)
   T dxa_m = T(1), dxa_r = T(1);
   if (sc.edge_strip) {
-    dxa_m = (a.dxa + sc.tg)[(long long)(x.mc - a.jsd) * nid + sc.lo_q];
-    dxa_r = (a.dxa + sc.tg)[x.orow + sc.lo_q];
+    const T *g = fp64 ? a.dxa + sc.tg : a.gex + sc.tgf;
+    dxa_m = g[(long long)(x.mc - a.jsd) * nid + sc.lo_q];
+    dxa_r = g[x.orow + sc.lo_q];
   }
   // cube-edge role of cell r-2 along the columns (uniform across the wave)
   const int role_y = sc.edge_rows ? edge_role(r - 2, a.js, a.je, a.npy, a.edge) : 0;
@@ -285,8 +288,8 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     else
       q_i = (qm * area_m + lv.yfx_prev * s.fy2_prev - yfx * fy2) * rray;
     // outer x-sweep of row r-3, inner x-sweep of row r, q_j of row r
-    const T fx = xsweep<SIDE>(q_i, cxm, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, hord, dxa_m);
-    const T fx2 = xsweep<SIDE>(qx, cxr, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, ord_in, dxa_r);
+    const T fx = xsweep<SIDE>(q_i, cxm, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, hord, dxa_m, sc.ec);
+    const T fx2 = xsweep<SIDE>(qx, cxr, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, ord_in, dxa_r, sc.ec);
     s.fx2_r[U] = fx2;
     const T fx2p = from_right(fx2);
     T q_j;
@@ -561,6 +564,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
   sc.col_out = lane >= NG && lane < NG + FT_BX && sc.i <= a.ie;
   sc.lane_in = sc.i >= a.is && sc.i <= a.ie;
   sc.edge_strip = EK >= 2 && a.edge && (i0 - NG <= 2 || i0 - NG + FT_LANES - 1 >= a.npx - 2);
+  if constexpr (sizeof(T) == 4) sc.ec = edge_coef_of(sc.edge_strip ? edge_role(sc.i, a.is, a.ie, a.npx, a.edge) : 0);
   sc.edge_rows = EK != 0;
   sc.corner_wg = EK >= 2 && a.corners != 0u && (strip == 0 || strip == a.nstrip - 1) && (vb == 0 || vb == a.nvblk - 1);
   sc.scale = nspl > 1;

@@ -62,6 +62,7 @@ struct FusedArgs {
   // metrics (one tile)
   const T *area, *rarea, *dxa, *dya, *dx, *dy, *sin_sg;
   const T *gxp, *gxm, *gyp, *gym;  // FP32 form: combined upwind area-flux factors
+  const T *gex;  // FP32 form: per-cell weights of the cube-edge face value (tile stride tgf)
   long long ms2;                   // metric slab (isd:ied, jsd:jed)
 };
 
@@ -237,6 +238,53 @@ __device__ __forceinline__ void edge_coef_side(T qm1, T q0, T qp1, T dmm1, T dm0
   }
 }
 
+// FP32 edge strips: a lane's cube-edge (bl, br) candidates as fixed linear
+// combinations of its neighbourhood,
+//   l = cl . (qm1, q0, dmm1, dm0, al0, efl),  r = cr . (q0, qp1, dmp1, dm0, alp1, efr),
+// with the coefficients of its edge role (the same sums as edge_coef_sel);
+// on: the lane has a role (its interior (bl, br) are replaced).
+struct EdgeCoef {
+  float l[6], r[6];
+  bool on;
+};
+
+__device__ __forceinline__ EdgeCoef edge_coef_of(int role) {
+  constexpr float s11 = K<float>::s11, s14 = K<float>::s14, s15 = K<float>::s15;
+  EdgeCoef e;
+#pragma unroll
+  for (int m = 0; m < 6; m++) e.l[m] = e.r[m] = 0.f;
+  e.on = role != 0;
+  switch (role) {
+    case 1:  // cell 0: s14 dm(-1) + s11 (q(-1) - q0); efr - q0
+      e.l[0] = s11; e.l[1] = -s11; e.l[2] = s14;
+      e.r[0] = -1.f; e.r[5] = 1.f;
+      break;
+    case 2:  // cell 1: efl - q0; s15 q0 + s11 q(+1) - s14 dm(+1) - q0
+      e.l[1] = -1.f; e.l[5] = 1.f;
+      e.r[0] = s15 - 1.f; e.r[1] = s11; e.r[2] = -s14;
+      break;
+    case 3:  // cell 2: s15 q(-1) + s11 q0 - s14 dm0 - q0; al(+1) - q0
+      e.l[0] = s15; e.l[1] = s11 - 1.f; e.l[3] = -s14;
+      e.r[0] = -1.f; e.r[4] = 1.f;
+      break;
+    case 4:  // cell npx-2: al0 - q0; s15 q(+1) + s11 q0 + s14 dm0 - q0
+      e.l[1] = -1.f; e.l[4] = 1.f;
+      e.r[0] = s11 - 1.f; e.r[1] = s15; e.r[3] = s14;
+      break;
+    case 5:  // cell npx-1: s15 q0 + s11 q(-1) + s14 dm(-1) - q0; efr - q0
+      e.l[0] = s11; e.l[1] = s15 - 1.f; e.l[2] = s14;
+      e.r[0] = -1.f; e.r[5] = 1.f;
+      break;
+    case 6:  // cell npx: efl - q0; s11 (q(+1) - q0) - s14 dm(+1)
+      e.l[1] = -1.f; e.l[5] = 1.f;
+      e.r[0] = -s11; e.r[1] = s11; e.r[2] = -s14;
+      break;
+    default:
+      break;
+  }
+  return e;
+}
+
 // (bl, br) of an interior cell for the monotone orders
 template <typename T>
 __device__ __forceinline__ void cell_coef(T qm2, T qm1, T q0, T qp1, T qp2, T dmm1, T dm0, T dmp1,
@@ -286,7 +334,7 @@ __device__ __forceinline__ T face_flux2(T c, T qL, T blL, T brL, T qR, T blR, T 
 // only the right one (the edge code then evaluates half the candidates).
 template <int SIDE = 0, typename T>
 __device__ __forceinline__ T xsweep(T q, T c, int i, int is, int ie, int npx, bool edge,
-                                    bool edge_strip, int iord, T dxl) {
+                                    bool edge_strip, int iord, T dxl, const EdgeCoef &ec) {
   const T qm1 = from_left(q), qp1 = from_right(q);
   const T dm = dm3(qm1, q, qp1);
   const T dmm1 = from_left(dm), dmp1 = from_right(dm);
@@ -300,14 +348,43 @@ __device__ __forceinline__ T xsweep(T q, T c, int i, int is, int ie, int npx, bo
   T bl, br;
   cell_coef(qm2, qm1, q, qp1, qp2, dmm1, dm, dmp1, al, alp1, iord, bl, br);
   if (edge_strip) {
-    // the left edge face of this cell is the right one of its left neighbour
-    const int role = edge_role(i, is, ie, npx, edge);
-    const T d1 = from_left(dxl), d3 = from_right(dxl), d4 = from_right(d3);
-    const T efr = edge_face(qm1, q, qp1, qp2, d1, dxl, d3, d4);
-    if constexpr (SIDE == 0)
-      edge_coef_sel(qm1, q, qp1, dmm1, dm, dmp1, al, alp1, role, from_left(efr), efr, bl, br);
-    else
-      edge_coef_side<SIDE>(qm1, q, qp1, dmm1, dm, dmp1, al, alp1, role, from_left(efr), efr, bl, br);
+    if constexpr (sizeof(T) == 4) {
+      // dxl: this cell's weight in the edge face value at the right face of
+      // the cell left of the cube edge (non-zero for the two cells either
+      // side of that face and their outer neighbours): the weighted sum of
+      // the four cells starting one to the left, then clipped to their range
+      const T p = dxl * q;
+      const T pp1 = from_right(p);
+      T efr = (from_left(p) + p) + (pp1 + from_right(pp1));
+      efr = fmx(efr, fmn(fmn3(qm1, q, qp1), qp2));
+      efr = fmn(efr, fmx(fmx3(qm1, q, qp1), qp2));
+      const T efl = from_left(efr);
+      // roles 1..3 never need al0 (l) or dm0 (r), roles 4..6 never dm0 (l) or alp1 (r)
+      T l = ec.l[0] * qm1 + ec.l[1] * q + ec.l[2] * dmm1 + ec.l[5] * efl;
+      T r = ec.r[0] * q + ec.r[1] * qp1 + ec.r[2] * dmp1 + ec.r[5] * efr;
+      if constexpr (SIDE != 2) {
+        l += ec.l[3] * dm;
+        r += ec.r[4] * alp1;
+      }
+      if constexpr (SIDE != 1) {
+        l += ec.l[4] * al;
+        r += ec.r[3] * dm;
+      }
+      pert_ppm1(q, l, r, 1);
+      if (ec.on) {
+        bl = l;
+        br = r;
+      }
+    } else {
+      // the left edge face of this cell is the right one of its left neighbour
+      const int role = edge_role(i, is, ie, npx, edge);
+      const T d1 = from_left(dxl), d3 = from_right(dxl), d4 = from_right(d3);
+      const T efr = edge_face(qm1, q, qp1, qp2, d1, dxl, d3, d4);
+      if constexpr (SIDE == 0)
+        edge_coef_sel(qm1, q, qp1, dmm1, dm, dmp1, al, alp1, role, from_left(efr), efr, bl, br);
+      else
+        edge_coef_side<SIDE>(qm1, q, qp1, dmm1, dm, dmp1, al, alp1, role, from_left(efr), efr, bl, br);
+    }
   }
   const T blm = from_left(bl), brm = from_left(br);
   return face_flux2(c, qm1, blm, brm, q, bl, br);
