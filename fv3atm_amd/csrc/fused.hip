@@ -13,7 +13,7 @@ namespace fv3 {
 template <int HO>
 static void go64(const FusedArgs<double> &a, const KindStreams &s) {
   const dim3 b(FT_LANES * 4);
-  int n[5];
+  int n[7];
   fused_grid_split(a, n);
   auto grid = [&](int c) { return dim3((unsigned)c, (unsigned)a.npz, (unsigned)a.ntiles); };
   if (n[0] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 0, 1>), grid(n[0]), b, 0, s.k[0], a);
@@ -21,6 +21,8 @@ static void go64(const FusedArgs<double> &a, const KindStreams &s) {
   if (n[2] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 2, 1>), grid(n[2]), b, 0, s.k[2], a);
   if (n[3] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 3, 1>), grid(n[3]), b, 0, s.k[2], a);
   if (n[4] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 4, 1>), grid(n[4]), b, 0, s.k[3], a);
+  if (n[5] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 5, 1>), grid(n[5]), b, 0, s.k[2], a);
+  if (n[6] > 0) hipLaunchKernelGGL((k_fused_tp<double, 4, 1, HO, 6, 1>), grid(n[6]), b, 0, s.k[3], a);
 }
 
 int launch_fused_f64(const FusedArgs<double> &a, const KindStreams &s) {

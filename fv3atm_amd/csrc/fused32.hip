@@ -12,7 +12,7 @@ namespace fv3 {
 template <int W, int NT, int HO, int OCC_E, int WE, int NTE, int OCC_L0>
 static void go2(const FusedArgs<float> &a, const KindStreams &s) {
   constexpr int OCC_L = OCC_L0 > 0 ? OCC_L0 : (NT > 1 ? 1 : (W >= 4 ? 12 / W : 3));
-  int n[5];
+  int n[7];
   fused_grid_split(a, n);
   auto grid = [&](int c) { return dim3((unsigned)c, (unsigned)a.npz, (unsigned)a.ntiles); };
   const dim3 bl(FT_LANES * W), be(FT_LANES * WE);
@@ -21,6 +21,8 @@ static void go2(const FusedArgs<float> &a, const KindStreams &s) {
   if (n[2] > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 2, OCC_E>), grid(n[2]), be, 0, s.k[2], a);
   if (n[3] > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 3, OCC_E>), grid(n[3]), be, 0, s.k[2], a);
   if (n[4] > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 4, OCC_E>), grid(n[4]), be, 0, s.k[3], a);
+  if (n[5] > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 5, OCC_E>), grid(n[5]), be, 0, s.k[2], a);
+  if (n[6] > 0) hipLaunchKernelGGL((k_fused_tp<float, WE, NTE, HO, 6, OCC_E>), grid(n[6]), be, 0, s.k[3], a);
 }
 
 template <int W, int NT, int OCC_E = 1, int WE = W, int NTE = NT, int OCC_L = 0>

@@ -19,6 +19,7 @@ template <typename T>
 struct StripCtx {
   int i, ic, j0, h, nix, niy, lo_q, lo_fx, lo_mx;  // lane column; block rows; lane offsets in the row layouts
   bool col_out, lane_in, edge_strip, edge_rows, corner_wg, scale;
+  bool gen_src;  // q sources by the general (halo, six-tile, corner) rules
   T frac;
   const T *cxk, *cyk, *mfxk, *mfyk;
   const double *dp1k;
@@ -57,6 +58,12 @@ struct RowIn {
   T qy[NT], qx[NT];
 };
 
+// q sources per input row of a kind: the strips with corner cells read the
+// y- and x-sweep sources of a cell separately
+constexpr int ek_qn(int ek) { return ek >= 2 && ek <= 4 ? 2 : 1; }
+// x-sweep cube-edge side of a kind: 1 left only, 2 right only, 0 either
+constexpr int ek_side(int ek) { return ek == 3 || ek == 5 ? 1 : (ek == 4 || ek == 6 ? 2 : 0); }
+
 template <typename T>
 struct RowIdx {
   int r, rc, m, mc, fc;
@@ -92,6 +99,16 @@ __device__ __forceinline__ void q_src(const FusedArgs<T> &a, const StripCtx<T> &
   const int nid = a.nid;
   // EK 1 blocks have every lane inside the compute domain, EK 0 every cell
   const bool inside = EK == 0 || ((EK == 1 || sc.lane_in) && x.row_in);  // EK >= 2: strips
+  if constexpr (EK >= 5) {  // edge strips, lean rows: halo lanes, no corner cell
+    py = (inside ? s.qin : s.qhalo) + x.orow + sc.lo_q;
+    if (a.ntiles > 1 && !inside) {
+      int ts, si, sj;
+      cube_halo_src(blockIdx.z + 1, sc.ic, x.rc, a.ie, ts, si, sj);
+      py = s.qsrc0 + (long long)(ts - 1) * a.tq + (long long)(sj - a.jsd) * nid + (si - a.isd);
+    }
+    px = py;
+    return;
+  }
   py = (inside ? s.qin : s.qhalo) + x.orow + sc.lo_q;
   px = py;
   if constexpr (EK == 1) {
@@ -102,7 +119,7 @@ __device__ __forceinline__ void q_src(const FusedArgs<T> &a, const StripCtx<T> &
       px = py;
     }
   }
-  if constexpr (EK >= 2) {
+  if constexpr (EK >= 2 && EK <= 4) {
     if (a.ntiles > 1 && !inside) {
       const int n = a.ie;
       const int tile = blockIdx.z + 1;
@@ -174,12 +191,12 @@ __device__ __forceinline__ void row_fetch(const FusedArgs<T> &a, const StripCtx<
   for (int n = 0; n < NT; n++) {
     if (!act[n]) continue;
     const T *py, *px;
-    if (sc.edge_rows)
+    if (sc.gen_src)
       q_src<T, 2>(a, sc, st[n], x, py, px);
     else
       q_src<T, 0>(a, sc, st[n], x, py, px);
     const T qy = *py;
-    const T qx = sc.edge_rows ? *px : qy;
+    const T qx = sc.gen_src ? *px : qy;
     in.qy[n] = qy;
     in.qx[n] = qx;
   }
@@ -345,8 +362,8 @@ __device__ __forceinline__ void wait_vm_lgkm0() {
 template <int W, int NT, int EK>
 __device__ __forceinline__ void ring_issue(const FusedArgs<float> &a, const StripCtx<float> &sc,
                                            const TrState<float> (&st)[NT],
-                                           LdsRing<W, NT, EK >= 2 ? 2 : 1> &ring, int wave, int t) {
-  constexpr int QN = EK >= 2 ? 2 : 1;
+                                           LdsRing<W, NT, ek_qn(EK)> &ring, int wave, int t) {
+  constexpr int QN = ek_qn(EK);
   const RowIdx<float> x = row_idx(a, sc, t);
   const int nid = a.nid;
   float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
@@ -447,8 +464,8 @@ __device__ __forceinline__ void ring_issue_lean(const FusedArgs<float> &a, const
 template <int W, int NT, int EK>
 __device__ __forceinline__ void ring_issue_mixed(const FusedArgs<float> &a, const StripCtx<float> &sc,
                                                  const TrState<float> (&st)[NT], const LeanRows<W> &lr,
-                                                 LdsRing<W, NT, EK >= 2 ? 2 : 1> &ring, int wave, int t) {
-  constexpr int QN = EK >= 2 ? 2 : 1;
+                                                 LdsRing<W, NT, ek_qn(EK)> &ring, int wave, int t) {
+  constexpr int QN = ek_qn(EK);
   float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
 #pragma unroll
   for (int m = 0; m < LeanRows<W>::M; m++)
@@ -507,8 +524,10 @@ __device__ __forceinline__ void ring_wait(int ahead) {
 // corner or halo code is compiled); 1: blocks of the interior strips that
 // reach a halo or cube-edge row (no per-lane edge or corner code); 2: the
 // strips that reach a halo or cube-edge column when no interior strip
-// exists; 3 / 4: those strips left / right of the interior strips (each can
-// only meet its own cube edge).
+// exists; 3 / 4: the top and bottom blocks of those strips left / right of
+// the interior strips (each can only meet its own cube edge); 5 / 6: their
+// lean rows (halo lanes and the x-direction cube-edge code, but no corner
+// cell, no y-direction edge code and one q source per cell).
 // OCC: minimum workgroups per CU asked of the register allocator.
 template <typename T, int W, int NT, int HO, int EK, int OCC>
 __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T> a) {
@@ -534,20 +553,23 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
       vb = bx / ns;
       j0 = vb == 0 ? a.js : a.lb + 1;
       h = vb == 0 ? a.lt - a.js : a.je - a.lb;
-    } else {
-      if constexpr (EK == 2) {  // every strip (no interior strip exists)
-        strip = bx % a.nstrip;
-        vb = bx / a.nstrip;
-      } else if constexpr (EK == 3) {  // strips left of the interior strips
-        strip = bx % a.s_lo;
-        vb = bx / a.s_lo;
-      } else {  // strips right of the interior strips
-        const int nr = a.nstrip - 1 - a.s_hi;
-        strip = a.s_hi + 1 + bx % nr;
-        vb = bx / nr;
-      }
+    } else if constexpr (EK == 2) {  // every strip (no interior strip exists)
+      strip = bx % a.nstrip;
+      vb = bx / a.nstrip;
       j0 = a.js + vb * a.H;
       h = min(a.H, a.je + 1 - j0);
+    } else {
+      // 3 / 5: strips left of the interior strips; 4 / 6: right of them
+      const int nside = EK == 3 || EK == 5 ? a.s_lo : a.nstrip - 1 - a.s_hi;
+      strip = (EK == 3 || EK == 5 ? 0 : a.s_hi + 1) + bx % nside;
+      vb = bx / nside;
+      if constexpr (EK <= 4) {  // the top and bottom blocks
+        j0 = vb == 0 ? a.js : a.lb + 1;
+        h = vb == 0 ? a.lt - a.js : a.je - a.lb;
+      } else {  // the lean rows
+        j0 = a.lt + vb * a.H;
+        h = min(a.H, a.lb + 1 - j0);
+      }
     }
     (void)ns;
   }
@@ -565,8 +587,10 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
   sc.lane_in = sc.i >= a.is && sc.i <= a.ie;
   sc.edge_strip = EK >= 2 && a.edge && (i0 - NG <= 2 || i0 - NG + FT_LANES - 1 >= a.npx - 2);
   if constexpr (sizeof(T) == 4) sc.ec = edge_coef_of(sc.edge_strip ? edge_role(sc.i, a.is, a.ie, a.npx, a.edge) : 0);
-  sc.edge_rows = EK != 0;
-  sc.corner_wg = EK >= 2 && a.corners != 0u && (strip == 0 || strip == a.nstrip - 1) && (vb == 0 || vb == a.nvblk - 1);
+  sc.edge_rows = EK != 0 && EK <= 4;
+  sc.gen_src = EK != 0;
+  sc.corner_wg = a.corners != 0u && (strip == 0 || strip == a.nstrip - 1) &&
+                 (EK == 2 ? (vb == 0 || vb == a.nvblk - 1) : (EK == 3 || EK == 4));
   sc.scale = nspl > 1;
   sc.frac = T(1) / T(nspl);
   sc.nix = a.ie - a.is + 2;
@@ -608,7 +632,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
     lv.yfx_prev = lv.mfy_prev = T(0);
     const bool any = act[0];
     if constexpr (sizeof(T) == 4) {
-      constexpr int QN = EK >= 2 ? 2 : 1;
+      constexpr int QN = ek_qn(EK);
       static_assert(LR_DEPTH == 4, "ring_wait covers up to two rows ahead");
       __shared__ LdsRing<W, NT, QN> ring;
       // the previous round's last reads of every slot are done before the
@@ -618,6 +642,8 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
       auto issue = [&](int t) {
         if constexpr (EK == 0) {
           ring_issue_lean<W, NT>(a, sc, st, lr, ring, wave, t);
+        } else if constexpr (EK >= 5) {  // lean rows: no row index clamps
+          ring_issue_mixed<W, NT, EK>(a, sc, st, lr, ring, wave, t);
         } else {
           // rows whose row indices do not clamp take the affine addresses
           // for the level arrays; the q sources keep their per-lane rules
@@ -638,7 +664,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
     if (any) {                                                                       \
       RowIn<T, NT> in;                                                               \
       ring_read<W, NT, QN>(ring, wave, lane, t, in);                                 \
-      row_step<T, NT, U, HO, EK == 3 ? 1 : (EK == 4 ? 2 : 0)>(a, sc, lv, st, act, t, in); \
+      row_step<T, NT, U, HO, ek_side(EK)>(a, sc, lv, st, act, t, in); \
     }                                                                                \
   }
         FV3_RROW(0) FV3_RROW(1) FV3_RROW(2) FV3_RROW(3) FV3_RROW(4) FV3_RROW(5)
@@ -656,7 +682,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
 #define FV3_ROW(U)                                                                   \
   if (any && t0 + U < nrow) {                                                        \
     if (t0 + U + 1 < nrow) row_fetch<T, NT>(a, sc, st, act, t0 + U + 1, buf[(U + 1) & 1]); \
-    row_step<T, NT, U, HO, EK == 3 ? 1 : (EK == 4 ? 2 : 0)>(a, sc, lv, st, act, t0 + U, buf[U & 1]); \
+    row_step<T, NT, U, HO, ek_side(EK)>(a, sc, lv, st, act, t0 + U, buf[U & 1]); \
   }                                                                                  \
   if (W > 1) __builtin_amdgcn_s_barrier();
       FV3_ROW(0) FV3_ROW(1) FV3_ROW(2) FV3_ROW(3) FV3_ROW(4) FV3_ROW(5)

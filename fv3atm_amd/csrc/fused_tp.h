@@ -71,13 +71,16 @@ struct FusedArgs {
 // strips when there is no interior strip, n[3] / n[4] the strips left / right
 // of the interior strips.
 template <typename T>
-inline void fused_grid_split(const FusedArgs<T> &a, int (&n)[5]) {
+inline void fused_grid_split(const FusedArgs<T> &a, int (&n)[7]) {
   const int ns = a.s_hi >= a.s_lo ? a.s_hi - a.s_lo + 1 : 0;
+  const int nl = ns > 0 ? a.s_lo : 0, nr = ns > 0 ? a.nstrip - 1 - a.s_hi : 0;
   n[0] = ns * a.nmid;
   n[1] = ns * 2;  // the top and bottom edge-row blocks of each interior strip
   n[2] = ns > 0 ? 0 : a.nstrip * a.nvblk;
-  n[3] = ns > 0 ? a.s_lo * a.nvblk : 0;
-  n[4] = ns > 0 ? (a.nstrip - 1 - a.s_hi) * a.nvblk : 0;
+  n[3] = nl * 2;  // the strips left of the interior strips: top and bottom blocks
+  n[4] = nr * 2;  // right of them: top and bottom blocks
+  n[5] = nl * a.nmid;  // left strips, lean rows
+  n[6] = nr * a.nmid;  // right strips, lean rows
 }
 
 // ---- cross-lane shifts: value of lane l-1 / l+1 (DPP wave_shr:1 / wave_shl:1)
