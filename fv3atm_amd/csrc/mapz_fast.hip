@@ -136,6 +136,40 @@ struct FieldArgs {
   const float *q_src, *pt_src;
   float *q_dst, *pt_dst;
   RemapCols cols;
+  // byte offsets of rbt gg dp1 bot lk from d4 (one carved block), and the
+  // byte sizes of that block (d4 .. lk) and of cut
+  unsigned o_rbt, o_gg, o_dp1, o_bot, o_lk, b_cols, b_cut;
+};
+
+// Raw buffer access: a wave-uniform descriptor per array, the lane's byte
+// offset in a VGPR and the level's byte offset in an SGPR, so that no load
+// needs per-lane 64-bit address arithmetic.  (Every array of a launch is
+// below 2 GiB; descriptor word 3 is the gfx9 raw 32-bit data format.)
+using Rsrc = __amdgpu_buffer_rsrc_t;
+
+__device__ __forceinline__ Rsrc make_rsrc(const void *p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float ldf(Rsrc r, unsigned vo, unsigned so) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ int ldi(Rsrc r, unsigned vo, unsigned so) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0);
+}
+__device__ __forceinline__ void stf(Rsrc r, unsigned vo, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, 0, 0);
+}
+
+// the column arrays of one launch: level k's value of the lane's column is
+// at lane offset vc (= 4 col) and scalar offset o_X + (k-1) lvl
+struct ColBuf {
+  Rsrc r, rc;  // d4 .. lk; cut
+  unsigned vc, lvl, o_rbt, o_gg, o_dp1, o_bot, o_lk;
+  __device__ __forceinline__ float d4(int k) const { return ldf(r, vc, (unsigned)(k - 1) * lvl); }
+  __device__ __forceinline__ float rbt(int k) const { return ldf(r, vc, o_rbt + (unsigned)(k - 1) * lvl); }
+  __device__ __forceinline__ float gg(int k) const { return ldf(r, vc, o_gg + (unsigned)(k - 1) * lvl); }
+  __device__ __forceinline__ float dp1(int k) const { return ldf(r, vc, o_dp1 + (unsigned)(k - 1) * lvl); }
+  __device__ __forceinline__ float bot(int m) const { return ldf(r, vc, o_bot + (unsigned)m * lvl); }
 };
 
 __device__ __forceinline__ void huynh2f(float a1, float &a2, float &a3, float gk, float gk1, float gk2,
@@ -154,11 +188,13 @@ struct TargetRow {
   int l;     // source layer of the top interface
 };
 
-__device__ __forceinline__ TargetRow load_target(const RemapCols &o, int k, long long col, long long nc) {
+// target row at the lane's level offset tvo (= (kk-1) lvl + vc); the four
+// dword loads at consecutive offsets compile to one dwordx4 buffer load
+__device__ __forceinline__ TargetRow load_target(const ColBuf &cb, unsigned tvo) {
   TargetRow t;
-  const long long off = (long long)(k - 1) * nc + col;
-  t.d = o.cut[off];
-  t.l = o.lk[off];
+  const unsigned v = tvo * 4u;
+  t.d = make_float4(ldf(cb.rc, v, 0), ldf(cb.rc, v + 4u, 0), ldf(cb.rc, v + 8u, 0), ldf(cb.rc, v + 12u, 0));
+  t.l = ldi(cb.r, tvo, cb.o_lk);
   return t;
 }
 
@@ -213,18 +249,20 @@ __device__ __forceinline__ void cell_limit(int c, int km, int iv, int akord, flo
 }
 
 // Mapping state of one (column, field): the open target kk (whose bottom
-// interface has been passed), its partial sum, and the row of target kk.
-// The row of the next target is loaded as soon as kk closes; it is first
-// needed a whole source layer later, which hides the load.
+// interface has been passed), its partial sum, the row of target kk, and
+// the lane's byte offsets of target kk in the column arrays (tvo) and in the
+// destination (dvo).  The row of the next target is loaded as soon as kk
+// closes; it is first needed a whole source layer later, which hides the load.
 struct MapState {
   int kk;
   float acc;
   TargetRow cur;
+  unsigned tvo, dvo;
 };
 
 __device__ __forceinline__ void map_layer(MapState &m, int c, int km, float a1, float a2, float a3,
-                                          float a4, float dp1c, const RemapCols &o, long long col,
-                                          long long nc, float *dst, long long s2, bool act) {
+                                          float a4, float dp1c, const ColBuf &cb, Rsrc dst, unsigned s2b,
+                                          bool act) {
   const float c0 = a2, c1 = 0.5f * (a4 + a3 - a2), c2 = a4 * (1.0f / 3.0f);
   bool bot_in = c == km;  // the bottom interface km+1 sits at s = 1 of layer km
   float sb = 1.f;
@@ -235,12 +273,14 @@ __device__ __forceinline__ void map_layer(MapState &m, int c, int km, float a1, 
       v = pmean(c0, c1, c2, s, sb);
     else
       v = (m.acc + m.cur.d.y * pmean(c0, c1, c2, s, 1.f)) * m.cur.d.w;
-    if (act) dst[(long long)(m.kk - 1) * s2] = v;
+    if (act) stf(dst, m.dvo, v);
     m.acc = 0.f;
     bot_in = true;
     sb = s;
     m.kk--;
-    if (m.kk >= 1) m.cur = load_target(o, m.kk, col, nc);
+    m.tvo -= cb.lvl;
+    m.dvo -= s2b;
+    if (m.kk >= 1) m.cur = load_target(cb, m.tvo);
   }
   if (m.kk >= 1) {
     if (!bot_in)
@@ -253,17 +293,18 @@ __device__ __forceinline__ void map_layer(MapState &m, int c, int km, float a1, 
 template <int B>
 __global__ void __launch_bounds__(RF_COLS * RF_WAVES) k_remap_fields(const FieldArgs a) {
   extern __shared__ float cp_lds[];  // [nb][256] eliminated values at block ends
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // scalar: descriptors stay in SGPRs
   // blocks are dealt round-robin to the 8 XCDs: renumber so that the field
   // groups of one column block run on one XCD, one after another
   const unsigned bid = blockIdx.x, nx = 8, q8 = a.nwg / nx, r8 = a.nwg % nx;
   const unsigned xcd = bid % nx, idx = bid / nx;
   const unsigned logical = xcd < r8 ? xcd * (q8 + 1) + idx : r8 * (q8 + 1) + (xcd - r8) * q8 + idx;
-  const int cb = (int)(logical / a.nfg), fg = (int)(logical % a.nfg);
+  const int cblk = (int)(logical / a.nfg), fg = (int)(logical % a.nfg);
   const int f = fg * RF_WAVES + wave;
   if (f >= a.nf) return;  // whole wave; no barriers below
   const long long nc = a.ncol;
-  const long long col_raw = (long long)cb * RF_COLS + lane;
+  const long long col_raw = (long long)cblk * RF_COLS + lane;
   const bool act = col_raw < nc;
   const long long col = act ? col_raw : nc - 1;
   const int ii = (int)(col % a.ni), jj = (int)(col / a.ni);
@@ -274,22 +315,30 @@ __global__ void __launch_bounds__(RF_COLS * RF_WAVES) k_remap_fields(const Field
   const int iv = is_pt ? 1 : 0;
   const int akord = is_pt ? a.kord_tm : a.kord_tr;
   const float qmin = is_pt ? a.t_min : 0.f;
-  const float *src = (is_pt ? a.pt_src : a.q_src + (long long)f * km * s2) + cell;
-  float *dst = (is_pt ? a.pt_dst : a.q_dst + (long long)f * km * s2) + cell;
-  const RemapCols &o = a.cols;
+  const unsigned s2b = (unsigned)s2 * 4u, fbytes = (unsigned)km * s2b;
+  const Rsrc rsrc = make_rsrc(is_pt ? a.pt_src : a.q_src + (long long)f * km * s2, fbytes);
+  const Rsrc rdst = make_rsrc(is_pt ? a.pt_dst : a.q_dst + (long long)f * km * s2, fbytes);
+  const unsigned vq = (unsigned)cell * 4u;  // the lane's cell in a field level
+  auto src = [&](int k) { return ldf(rsrc, vq, (unsigned)(k - 1) * s2b); };
+  ColBuf cb;
+  cb.r = make_rsrc(a.cols.d4, a.b_cols);
+  cb.rc = make_rsrc(a.cols.cut, a.b_cut);
+  cb.vc = (unsigned)col * 4u;
+  cb.lvl = (unsigned)nc * 4u;
+  cb.o_rbt = a.o_rbt; cb.o_gg = a.o_gg; cb.o_dp1 = a.o_dp1; cb.o_bot = a.o_bot; cb.o_lk = a.o_lk;
   float *cp = cp_lds + tid;
 
   // ---- downward elimination; keep the value at every block end
-  float a_prev = src[0];
-  float qp = (o.d4[col] * a_prev + src[s2]) * o.rbt[col];
+  float a_prev = src(1);
+  float qp = (cb.d4(1) * a_prev + src(2)) * cb.rbt(1);
   for (int k0 = 2; k0 <= km; k0 += B) {
     float av[B], dv[B], rv[B];
 #pragma unroll
     for (int j = 0; j < B; j++) {
       const int k = min(k0 + j, km);
-      av[j] = src[(long long)(k - 1) * s2];
-      dv[j] = o.d4[(long long)(k - 1) * nc + col];
-      rv[j] = o.rbt[(long long)(k - 1) * nc + col];
+      av[j] = src(k);
+      dv[j] = cb.d4(k);
+      rv[j] = cb.rbt(k);
     }
 #pragma unroll
     for (int j = 0; j < B; j++) {
@@ -302,8 +351,8 @@ __global__ void __launch_bounds__(RF_COLS * RF_WAVES) k_remap_fields(const Field
     }
   }
   // qp = Q'(km); bottom row gives the raw interface value Q(km+1)
-  const float a_km = a_prev, a_km1 = src[(long long)(km - 2) * s2];
-  const float qbot = (o.bot[col] * a_km + a_km1 - o.bot[nc + col] * qp) * o.bot[2 * nc + col];
+  const float a_km = a_prev, a_km1 = src(km - 1);
+  const float qbot = (cb.bot(0) * a_km + a_km1 - cb.bot(1) * qp) * cb.bot(2);
 
   // ---- upward pass: window state after position p+1
   float A_m1 = a_km1, A_0 = a_km, A_p1 = 0.f, A_p2 = 0.f, A_p3 = 0.f;
@@ -313,7 +362,9 @@ __global__ void __launch_bounds__(RF_COLS * RF_WAVES) k_remap_fields(const Field
   MapState ms;
   ms.kk = km;
   ms.acc = 0.f;
-  ms.cur = load_target(o, km, col, nc);
+  ms.tvo = (unsigned)(km - 1) * cb.lvl + cb.vc;
+  ms.dvo = (unsigned)(km - 1) * s2b + vq;
+  ms.cur = load_target(cb, ms.tvo);
 
   for (int b = a.nb - 1; b >= 0; b--) {
     const int k0 = b * B + 1;
@@ -323,16 +374,15 @@ __global__ void __launch_bounds__(RF_COLS * RF_WAVES) k_remap_fields(const Field
 #pragma unroll
     for (int j = 0; j < B + 2; j++) {
       const int k = min(max(k0 - 2 + j, 1), km);
-      av[j] = src[(long long)(k - 1) * s2];
+      av[j] = src(k);
     }
 #pragma unroll
     for (int j = 0; j < B; j++) {
       const int k = min(k0 + j, km);
-      const long long off = (long long)(k - 1) * nc + col;
-      dv[j] = o.d4[off];
-      rv[j] = o.rbt[off];
-      gv[j] = o.gg[off];
-      pv[j] = o.dp1[off];
+      dv[j] = cb.d4(k);
+      rv[j] = cb.rbt(k);
+      gv[j] = cb.gg(k);
+      pv[j] = cb.dp1(k);
     }
     // regenerate Q'(k0 .. k0+cnt-1)
     float qv[B];
@@ -385,8 +435,8 @@ __global__ void __launch_bounds__(RF_COLS * RF_WAVES) k_remap_fields(const Field
           float a1c = A_p1, a2 = QL1, a3 = QL2, a4;
           cell_limit(c, km, iv, akord, qmin, a1c, a2, a3, a4, e0, E1, E2, A_0 - A_m1, A_p1 - A_0,
                      A_p2 - A_p1, A_p3 - A_p2);
-          const float dpc = j + 1 < B ? pv[j + 1] : o.dp1[(long long)(c - 1) * nc + col];
-          map_layer(ms, c, km, A_p1, a2, a3, a4, dpc, o, col, nc, dst, s2, act);
+          const float dpc = j + 1 < B ? pv[j + 1] : cb.dp1(c);
+          map_layer(ms, c, km, A_p1, a2, a3, a4, dpc, cb, rdst, s2b, act);
         }
         qraw_next = qr;
         QL2 = QL1;
@@ -411,7 +461,7 @@ __global__ void __launch_bounds__(RF_COLS * RF_WAVES) k_remap_fields(const Field
     } else {
       a4 = 3.f * (2.f * a1 - (a2 + a3));
     }
-    map_layer(ms, 1, km, a1, a2, a3, a4, o.dp1[col], o, col, nc, dst, s2, act);
+    map_layer(ms, 1, km, a1, a2, a3, a4, cb.dp1(1), cb, rdst, s2b, act);
   }
 }
 
@@ -484,6 +534,20 @@ int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_d
   fa.nfg = (fa.nf + RF_WAVES - 1) / RF_WAVES; fa.kord_tr = kord_tr; fa.kord_tm = kord_tm;
   fa.ncol = ncol; fa.s2 = d.s2; fa.t_min = (float)t_min;
   fa.q_src = q_src; fa.pt_src = pt_src; fa.q_dst = q_dst; fa.pt_dst = pt_dst; fa.cols = cols;
+  {
+    const char *b = (const char *)cols.d4;
+    const long long cut = (long long)sizeof(float4) * km * ncol, span = (const char *)cols.lk - b + 4LL * km * ncol;
+    const long long fld = 4LL * km * d.s2;
+    if (span >= (1LL << 31) || cut >= (1LL << 31) || fld >= (1LL << 31))
+      return set_error("fast remap: a tile's column data or field level set exceeds 2 GiB");
+    fa.o_rbt = (unsigned)((const char *)cols.rbt - b);
+    fa.o_gg = (unsigned)((const char *)cols.gg - b);
+    fa.o_dp1 = (unsigned)((const char *)cols.dp1 - b);
+    fa.o_bot = (unsigned)((const char *)cols.bot - b);
+    fa.o_lk = (unsigned)((const char *)cols.lk - b);
+    fa.b_cols = (unsigned)span;
+    fa.b_cut = (unsigned)cut;
+  }
   const long long ncb = (ncol + RF_COLS - 1) / RF_COLS;
   const long long nwg = ncb * fa.nfg;
   if (nwg > 0x7fffffffLL) return set_error("remap grid too large");
