@@ -294,8 +294,16 @@ __device__ __forceinline__ void cell_coef(T qm2, T qm1, T q0, T qp1, T qp2, T dm
                                           T al0, T alp1, int iord, T &bl, T &br) {
   if (iord == 8 || iord == 11) {
     T xt = iord == 8 ? T(2) * dm0 : K<T>::ppm_fac * dm0;
-    bl = -fsign(fmn(fabs(xt), fabs(al0 - q0)), xt);
-    br = fsign(fmn(fabs(xt), fabs(alp1 - q0)), xt);
+    if constexpr (sizeof(T) == 4) {
+      // -sign(xt) min(|xt|, |b|) is -xt clamped to [-|b|, |b|] (the same
+      // value, signed zeros aside): one median-of-three per side
+      const T b = al0 - q0, c = alp1 - q0;
+      bl = __builtin_amdgcn_fmed3f(-xt, -fabs(b), fabs(b));
+      br = __builtin_amdgcn_fmed3f(xt, -fabs(c), fabs(c));
+    } else {
+      bl = -fsign(fmn(fabs(xt), fabs(al0 - q0)), xt);
+      br = fsign(fmn(fabs(xt), fabs(alp1 - q0)), xt);
+    }
   } else {
     bl = al0 - q0;
     br = alp1 - q0;
@@ -316,19 +324,20 @@ __device__ __forceinline__ void cell_coef(T qm2, T qm1, T q0, T qp1, T qp2, T dm
 }
 
 // Upwind flux at a face with Courant number c, from the cells on its left
-// (qL, blL, brL) and right (qR, blR, brR).  Upstream writes the two branches
+// (qL, brL and sL = blL + brL) and right (qR, blR and sR = blR + brR).
+// Upstream writes the two branches
 //   c > 0:  qL + (1 - c) (brL - c (blL + brL))
 //   else:   qR + (1 + c) (blR + c (blR + brR));
-// with a = |c| both are  qu + (1 - a) (x - a (blu + bru))  where u is the
-// upwind cell and x its downwind-side perturbation: 1 - a equals 1 + c and
-// a S equals -(c S) exactly, so this branch-free form rounds the same.
+// with a = |c| both are  qu + (1 - a) (x - a su)  where u is the upwind cell,
+// x its downwind-side perturbation and su its sum: 1 - a equals 1 + c and
+// a S equals -(c S) exactly, so this branch-free form rounds the same, and a
+// cell's sum bl + br is formed once for the two faces that may use it.
 template <typename T>
-__device__ __forceinline__ T face_flux2(T c, T qL, T blL, T brL, T qR, T blR, T brR) {
+__device__ __forceinline__ T face_flux_s(T c, T qL, T brL, T sL, T qR, T blR, T sR) {
   const bool pos = c > T(0);
-  const T qu = pos ? qL : qR, blu = pos ? blL : blR, bru = pos ? brL : brR;
-  const T x = pos ? bru : blu;
+  const T qu = pos ? qL : qR, su = pos ? sL : sR, x = pos ? brL : blR;
   const T ac = fabs(c);
-  return qu + (T(1) - ac) * (x - ac * (blu + bru));
+  return qu + (T(1) - ac) * (x - ac * su);
 }
 
 // x-direction PPM across the lanes of a row: flux at the face left of each
@@ -389,8 +398,8 @@ __device__ __forceinline__ T xsweep(T q, T c, int i, int is, int ie, int npx, bo
         edge_coef_side<SIDE>(qm1, q, qp1, dmm1, dm, dmp1, al, alp1, role, from_left(efr), efr, bl, br);
     }
   }
-  const T blm = from_left(bl), brm = from_left(br);
-  return face_flux2(c, qm1, blm, brm, q, bl, br);
+  const T sum = bl + br;
+  return face_flux_s(c, qm1, from_left(br), from_left(sum), q, bl, sum);
 }
 
 // Sliding y-direction PPM over a column: fed one cell per row, yields the
@@ -401,7 +410,7 @@ struct YPipe {
   T w[6];   // q of rows, slot row%6
   T dm[6];  // dm of cells
   T al[6];  // face values
-  T bl[6], br[6];
+  T bl[6], br[6], bs[6];  // bs = bl + br
 };
 
 // One step at input row r; U is the slot of row r.  role/dd: the cube-edge
@@ -422,7 +431,8 @@ __device__ __forceinline__ T ypipe_step(YPipe<T> &p, T qnew, T c, int iord, int 
     edge_coef_sel(p.w[s3], p.w[s2], p.w[s1], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2], p.al[s1], role, efl,
                   efr, p.bl[s2], p.br[s2]);
   }
-  return face_flux2(c, p.w[s3], p.bl[s3], p.br[s3], p.w[s2], p.bl[s2], p.br[s2]);
+  p.bs[s2] = p.bl[s2] + p.br[s2];
+  return face_flux_s(c, p.w[s3], p.br[s3], p.bs[s3], p.w[s2], p.bl[s2], p.bs[s2]);
 }
 
 }  // namespace fv3
