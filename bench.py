@@ -317,8 +317,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="first run a 1 GiB copy kernel (reference for HBM byte counters)")
-    ap.add_argument("--launch", type=int, default=0,
-                    help="fused launch configuration (0: the library default); -1 tunes over all "
+    ap.add_argument("--launch", type=int, default=None,
+                    help="fused launch configuration (default: the library's, 9); -1 tunes over all "
                          "of them and the rows per block first")
     ap.add_argument("--rows", type=int, default=0, help="rows per block (0: the library default, 60)")
     ap.add_argument("--remap", type=int, default=0, help="remap kernel (0: the library default)")
@@ -424,7 +424,7 @@ def main():
         torch.cuda.empty_cache()
         d = device_inputs(case, torch, torch.float32)
     fa.set_timing(ctx, True)
-    launch, rows = args.launch, args.rows
+    launch, rows = (fa.DEFAULT_LAUNCH_CFG if args.launch is None else args.launch), args.rows
     if launch < 0:  # untimed tuning of the launch shape, then rows per block
         best = None
         for cfg in range(fa.N_LAUNCH_CFG):

@@ -224,9 +224,13 @@ def set_pipeline(ctx: Context, tracers_per_chunk: int = 0):
     check(_ffi.lib().fv3_ctx_set_pipeline(ctx.handle, int(tracers_per_chunk)), "fv3_ctx_set_pipeline")
 
 
-def set_launch(ctx: Context, cfg: int = 0, rows: int = 0, remap: int = 0):
-    """Kernel choices of the FP32 step: transport launch shape (cfg, rows;
-    results unchanged) and remap kernel (0 column-shared, 1 per column)."""
+DEFAULT_LAUNCH_CFG = 9
+
+
+def set_launch(ctx: Context, cfg: int = DEFAULT_LAUNCH_CFG, rows: int = 0, remap: int = 0):
+    """Kernel choices of the FP32 step: transport launch shape (cfg, the
+    library default 9; rows) -- results unchanged -- and remap kernel (0
+    column-shared, 1 per column)."""
     check(_ffi.lib().fv3_ctx_set_launch(ctx.handle, cfg, rows, remap), "fv3_ctx_set_launch")
 
 

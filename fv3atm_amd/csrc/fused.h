@@ -6,9 +6,6 @@
 
 namespace fv3 {
 
-// launch configurations (waves per workgroup x tracers per wave); cfg 0 is
-// the default, the others are the autotuner's candidates
-constexpr int FUSED32_NCFG = 15;
 // streams of the kernel kinds (lean, edge rows, edge strips / left strips,
 // right strips)
 struct KindStreams {

@@ -45,6 +45,12 @@ struct GridDev {
   long long s2, sdx, sdy;  // tile strides of (isd:ied,jsd:jed), dx, dy
 };
 
+// FP32 fused launch configurations (waves per workgroup x tracers per wave
+// per kernel kind); the default: 4 x 2 for the interior kernel, 4 x 1 at
+// three workgroups per CU for the edge kernels
+constexpr int FUSED32_NCFG = 15;
+constexpr int FUSED32_DEFAULT_CFG = 9;
+
 struct Ctx {
   Dims d;
   int npz = 0, ntiles = 1;
@@ -72,7 +78,7 @@ struct Ctx {
   // optional phase timing of fv3_step_dev (hipEvents on the step's stream)
   bool timing = false;
   // fused-kernel launch shape: configuration id and interior rows per block
-  int fused_cfg = 0, fused_h = 0;
+  int fused_cfg = FUSED32_DEFAULT_CFG, fused_h = 0;
   // FP32 step remap: 0 column-shared fast kernel, 1 one thread per column
   int remap_mode = 0;
   // fused step's sub-cycling: per-level nsplit(k) (tracer_2d_1L) or one

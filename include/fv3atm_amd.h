@@ -137,7 +137,9 @@ int fv3_step_dev(fv3_ctx *ctx, int form, int nq, const void *q_in, void *q_out, 
 int fv3_ctx_set_timing(fv3_ctx *ctx, int on);
 /* Kernel choices of the FP32 step (no counterpart upstream): cfg selects
  * the fused transport launch shape, waves per workgroup x tracers per wave
- * (0..14; 0 = the default, 4 x 2); rows the interior rows per block (0 = 60);
+ * (0..14; the default is 9: 4 waves x 2 tracers for the interior kernel, 4 x 1
+ * at three workgroups per CU for the edge kernels); rows the interior rows
+ * per block (0 = 60);
  * remap the vertical remap kernel, 0 = column-shared fast kernel (default),
  * 1 = one thread per column.  Transport results do not depend on cfg/rows;
  * the two remap kernels agree to FP32 rounding. */
