@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-tracer updates/s C768L127x30 tracers: FP32-batched vs plain FP64 @≤1e-5; %HBM"
 HBM_PEAK = 8.0e12
+VALU_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions/s: 256 CUs x 4 SIMDs, 4 cycles each
 
 
 def log(*a):
@@ -126,24 +127,34 @@ def run_graph(fa, ctx, d, case, steps, warmup, cap, torch):
     return ms, bad
 
 
-def profiled_traffic(n, npz, nq, cfg, rows):
-    """HBM bytes per transport launch measured by rocprofv3 counters for this
-    workload and launch shape: the 'bench-traffic:' line of a summary under
-    profiles/ (FETCH_SIZE + WRITE_SIZE of every k_fused_tp dispatch of one
-    step, each scaled by the same counter's reading for a copy kernel of known
-    size, divided by the step's sub-cycle passes).  None when no summary
-    matches."""
+def profiled(tag, field, n, npz, nq, cfg, rows):
+    """A per-launch figure of the transport that rocprofv3 counters measured
+    for this workload and launch shape: field of the '<tag>:' line of a
+    summary under profiles/ whose keys match.  (None, None) when none does."""
     import glob
     want = dict(n=str(n), npz=str(npz), nq=str(nq), cfg=str(cfg), rows=str(rows))
     for p in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         with open(p) as f:
             for line in f:
-                if not line.startswith("bench-traffic:"):
+                if not line.startswith(tag + ":"):
                     continue
                 kv = dict(t.split("=", 1) for t in line.split()[1:] if "=" in t)
                 if all(kv.get(k) == v for k, v in want.items()):
-                    return int(float(kv["bytes_per_pass"])), os.path.relpath(p, ROOT)
+                    return int(float(kv[field])), os.path.relpath(p, ROOT)
     return None, None
+
+
+def profiled_traffic(n, npz, nq, cfg, rows):
+    """HBM bytes per transport launch: FETCH_SIZE + WRITE_SIZE of every
+    k_fused_tp dispatch of one step, each scaled by the same counter's
+    reading for a copy kernel of known size, divided by the step's passes."""
+    return profiled("bench-traffic", "bytes_per_pass", n, npz, nq, cfg, rows)
+
+
+def profiled_valu(n, npz, nq, cfg, rows):
+    """VALU wave instructions per transport launch (SQ_INSTS_VALU of every
+    k_fused_tp dispatch of one step divided by the step's passes)."""
+    return profiled("bench-valu", "valu_per_pass", n, npz, nq, cfg, rows)
 
 
 def cpu_baseline(case, nq=2):
@@ -478,6 +489,14 @@ def main():
     out["launches_per_step"] = cap
     if tsrc:
         out["traffic_source"] = tsrc
+    # the bound that holds: VALU issue (wave64 instructions of the launch's
+    # kernels, from the committed counter summary, over the live launch time)
+    valu, vsrc = profiled_valu(N, L, Q, launch, rows or 60)
+    if valu:
+        rate = valu * cap / ta
+        out["valu_issue"] = dict(achieved=round(rate / 1e9, 1), peak=round(VALU_PEAK / 1e9, 1),
+                                 unit="G wave-instr/s", frac=round(rate / VALU_PEAK, 4),
+                                 instr_per_launch=valu, source=vsrc)
     out_line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-tracer updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),

@@ -11,6 +11,14 @@
       wall span (first start to last end) of every G consecutive dispatches
       whose name contains K, ordered by start: one launch made of G kernels
       that may run concurrently (mean, min, max)
+  python -m fv3atm_amd.rocprof_summary valu SQ_DIR TRACE_DIR --kernel K --launches L
+          [--only S] [--tag "k=v ..."]
+      per-kernel VALU issue rate: the mean SQ_INSTS_VALU per dispatch of the
+      counter run SQ_DIR over the mean duration per dispatch of the kernel
+      trace TRACE_DIR (a run whose kernels do not overlap), against the MI355X
+      issue peak (256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64
+      instruction); then the VALU instructions of every dispatch whose name
+      contains K summed and divided by L launches, as a 'bench-valu:' line
   python -m fv3atm_amd.rocprof_summary counters DIR [--calib-bytes B]
       per-kernel mean of every collected counter per dispatch; with
       --calib-bytes, the counter of the calibration kernel (the one whose
@@ -126,9 +134,29 @@ def calibrated_sum(d, counter, kernel, calib_name, calib_bytes):
     return tot, cnt, factor
 
 
+VALU_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second, MI355X
+
+
+def valu_table(sq_dir, trace_dir):
+    """[(kernel, mean ms, VALU, SALU, VALU G/s, fraction of peak)] for every
+    kernel with both a SQ_INSTS_VALU count and a trace duration."""
+    agg = counter_means(sq_dir)
+    st = kernel_stats(trace_dir)
+    out = []
+    for (n, cn), (c, v) in sorted(agg.items()):
+        if cn != "SQ_INSTS_VALU" or n not in st:
+            continue
+        calls, tot = st[n]
+        dur = tot / calls * 1e-9
+        valu = v / c
+        sc, sv = agg.get((n, "SQ_INSTS_SALU"), (1, 0.0))
+        out.append((n, dur * 1e3, valu, sv / sc, valu / dur / 1e9, valu / dur / VALU_PEAK))
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "counters", "traffic", "spans"])
+    ap.add_argument("mode", choices=["stats", "counters", "traffic", "spans", "valu"])
     ap.add_argument("dir")
     ap.add_argument("dir2", nargs="?")
     ap.add_argument("--kernel", default="k_fused_tp")
@@ -138,6 +166,7 @@ def main(argv=None):
     ap.add_argument("--calib-bytes", type=float, default=0.0)
     ap.add_argument("--calib-name", default="bitwise_not")
     ap.add_argument("--top", type=int, default=12)
+    ap.add_argument("--only", default="", help="valu: list only kernels whose name contains this")
     a = ap.parse_args(argv)
     w = sys.stdout.write
     if a.mode == "stats":
@@ -153,6 +182,17 @@ def main(argv=None):
             raise SystemExit(f"no dispatches of {a.kernel}")
         w(f"{len(sp)} launches of {a.group} `{a.kernel}` kernels: span mean {sum(sp) / len(sp) / 1e6:.3f} ms, "
           f"min {min(sp) / 1e6:.3f} ms, max {max(sp) / 1e6:.3f} ms\n")
+        return
+    if a.mode == "valu":
+        rows = valu_table(a.dir, a.dir2)
+        w("| kernel | mean ms | VALU | SALU | VALU G/s | VALU busy |\n|---|---:|---:|---:|---:|---:|\n")
+        for n, ms, v, sv, g, f in rows:
+            if a.only in n:
+                w(f"| `{clip(n)}` | {ms:.3f} | {v:.4e} | {sv:.4e} | {g:.0f} | {100 * f:.0f}% |\n")
+        agg = counter_means(a.dir)
+        tot = sum(v for (n, cn), (c, v) in agg.items() if cn == "SQ_INSTS_VALU" and a.kernel in n)
+        w(f"VALU instructions of `{a.kernel}` kernels per launch ({a.launches} launches): {tot / a.launches:.4e}\n")
+        w(f"bench-valu: {a.tag} valu_per_pass={tot / a.launches:.0f}\n")
         return
     if a.mode == "traffic":
         f, nf, ff = calibrated_sum(a.dir, "FETCH_SIZE", a.kernel, a.calib_name, a.calib_bytes)

@@ -71,3 +71,27 @@ def test_spans_of_concurrent_groups(tmp_path):
            [[k.format(0), 100, 400], [k.format(1), 150, 300], [k.format(0), 500, 700], [k.format(1), 510, 750],
             ["void other(int)", 0, 1000]])
     assert rs.spans(str(tmp_path), "k_fused_tp", 2) == [300.0, 250.0]
+
+
+def test_valu_rates_and_bench_line(tmp_path):
+    k1 = "void fv3::k_fused_tp<float, 4, 2, 8, 0, 1>(fv3::FusedArgs<float>)"
+    k2 = "void fv3::k_fused_tp<float, 4, 1, 8, 5, 3>(fv3::FusedArgs<float>)"
+    sq, tr = tmp_path / "sq", tmp_path / "tr"
+    sq.mkdir()
+    tr.mkdir()
+    # k1: two dispatches of 6.144e8 VALU each, 1 ms each -> 614.4 G/s = the issue peak
+    _write(sq / "1_counter_collection.csv", ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"],
+           [[1, k1, "SQ_INSTS_VALU", 6.144e8], [2, k1, "SQ_INSTS_VALU", 6.144e8], [1, k1, "SQ_INSTS_SALU", 1e8],
+            [2, k1, "SQ_INSTS_SALU", 1e8], [3, k2, "SQ_INSTS_VALU", 3.072e8]])
+    _write(tr / "1_kernel_trace.csv", ["Kernel_Name", "Start_Timestamp", "End_Timestamp"],
+           [[k1, 0, 1000000], [k1, 2000000, 3000000], [k2, 4000000, 6000000]])
+    rows = {r[0]: r for r in rs.valu_table(str(sq), str(tr))}
+    n1 = "fv3::k_fused_tp<float, 4, 2, 8, 0, 1>"
+    assert abs(rows[n1][1] - 1.0) < 1e-12 and abs(rows[n1][5] - 1.0) < 1e-12
+    assert abs(rows["fv3::k_fused_tp<float, 4, 1, 8, 5, 3>"][5] - 0.25) < 1e-12
+    out = io.StringIO()
+    with contextlib.redirect_stdout(out):
+        rs.main(["valu", str(sq), str(tr), "--kernel", "k_fused_tp", "--launches", "2", "--tag", "n=8"])
+    # (2 x 6.144e8 + 3.072e8) / 2 launches
+    line = [l for l in out.getvalue().splitlines() if l.startswith("bench-valu:")][0]
+    assert line == "bench-valu: n=8 valu_per_pass=768000000"
