@@ -336,7 +336,9 @@ def main():
     ap.add_argument("--serial-kinds", action="store_true",
                     help="run the transport's edge kernels after the lean kernel on one stream")
     ap.add_argument("--graph", action="store_true",
-                    help="time the step as hipGraph replays (graph-safe mode)")
+                    help="time the step as hipGraph replays (graph-safe mode): the default")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager steps only, not hipGraph replays of the captured step")
     ap.add_argument("--sphere", action="store_true",
                     help="the six-face C384 L127 Q20 sphere instead of the C768 tile")
     ap.add_argument("--pipeline", type=int, default=None,
@@ -345,6 +347,7 @@ def main():
     ap.add_argument("--mixed", action="store_true",
                     help="the C384 L64 Q10 FP32-against-FP64 error report instead of the C768 tile")
     args = ap.parse_args()
+    args.graph = not args.eager  # the workload runs as a captured step under hipGraph
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "across GPUs the path would shard one cube tile per GPU, which is not built",
