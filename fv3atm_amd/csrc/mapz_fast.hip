@@ -260,27 +260,44 @@ struct MapState {
   unsigned tvo, dvo;
 };
 
+// target kk's top interface lies in the current source layer: its mean is
+// complete (the layer's piece above the bottom edge sb, plus what it already
+// held); store it and move to target kk-1
+__device__ __forceinline__ void close_target(MapState &m, float c0, float c1, float c2, bool &bot_in,
+                                             float &sb, const ColBuf &cb, Rsrc dst, unsigned s2b,
+                                             bool act) {
+  const float s = m.cur.d.x;
+  float v;
+  if (bot_in)
+    v = pmean(c0, c1, c2, s, sb);
+  else
+    v = (m.acc + m.cur.d.y * pmean(c0, c1, c2, s, 1.f)) * m.cur.d.w;
+  if (act) stf(dst, m.dvo, v);
+  m.acc = 0.f;
+  bot_in = true;
+  sb = s;
+  m.kk--;
+  m.tvo -= cb.lvl;
+  m.dvo -= s2b;
+  if (m.kk >= 1) m.cur = load_target(cb, m.tvo);
+}
+
 __device__ __forceinline__ void map_layer(MapState &m, int c, int km, float a1, float a2, float a3,
                                           float a4, float dp1c, const ColBuf &cb, Rsrc dst, unsigned s2b,
                                           bool act) {
   const float c0 = a2, c1 = 0.5f * (a4 + a3 - a2), c2 = a4 * (1.0f / 3.0f);
   bool bot_in = c == km;  // the bottom interface km+1 sits at s = 1 of layer km
   float sb = 1.f;
-  while (m.kk >= 1 && m.cur.l == c) {
-    const float s = m.cur.d.x;
-    float v;
-    if (bot_in)
-      v = pmean(c0, c1, c2, s, sb);
-    else
-      v = (m.acc + m.cur.d.y * pmean(c0, c1, c2, s, 1.f)) * m.cur.d.w;
-    if (act) stf(dst, m.dvo, v);
-    m.acc = 0.f;
-    bot_in = true;
-    sb = s;
-    m.kk--;
-    m.tvo -= cb.lvl;
-    m.dvo -= s2b;
-    if (m.kk >= 1) m.cur = load_target(cb, m.tvo);
+  // a source layer holds the top interfaces of zero, one or (rarely) more
+  // targets: the first two closings as nested conditionals, the rest in a
+  // loop that only a column with a third one enters (a loop around every
+  // closing costs register copies of the whole state at each entry and exit)
+  if (m.kk >= 1 && m.cur.l == c) {
+    close_target(m, c0, c1, c2, bot_in, sb, cb, dst, s2b, act);
+    if (m.kk >= 1 && m.cur.l == c) {
+      close_target(m, c0, c1, c2, bot_in, sb, cb, dst, s2b, act);
+      while (m.kk >= 1 && m.cur.l == c) close_target(m, c0, c1, c2, bot_in, sb, cb, dst, s2b, act);
+    }
   }
   if (m.kk >= 1) {
     if (!bot_in)
