@@ -87,7 +87,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   T *q_tmp = nullptr, *pt_src = nullptr;
   std::vector<RemapCols> cols(nt);  // column data of every tile
   if (fast_remap) {
-    if (remap_fast_check(npz, kord_tr, kord_tm, pt != nullptr)) return 1;
+    if (remap_fast_check(npz, (long long)ni * nj, d.s2, kord_tr, kord_tm, pt != nullptr)) return 1;
     const size_t b_q = ((sizeof(T) * qsize) + 255) & ~(size_t)255;
     const size_t b_pt = pt ? ((sizeof(T) * d.s2 * npz * nt) + 255) & ~(size_t)255 : 0;
     const size_t b_col = remap_cols_bytes(npz, (long long)ni * nj);

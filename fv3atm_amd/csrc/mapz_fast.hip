@@ -514,6 +514,17 @@ RemapCols remap_cols_carve(void *base, int km, long long ncol) {
   return c;
 }
 
+// the field kernel addresses a tile's column data (d4 .. lk, and cut) and one
+// field's levels with 32-bit byte offsets
+static int check_sizes(int km, long long ncol, long long s2) {
+  auto al = [](long long b) { return (b + 255) & ~255LL; };
+  const long long span = 4 * al(4LL * km * ncol) + al(12LL * ncol) + 4LL * km * ncol;
+  if (span >= (1LL << 31) || 16LL * km * ncol >= (1LL << 31) || 4LL * km * s2 >= (1LL << 31))
+    return set_error("fast remap: a tile's column data or field level set exceeds 2 GiB "
+                     "(the one-thread-per-column remap, fv3_ctx_set_launch remap = 1, has no such limit)");
+  return 0;
+}
+
 static int check_orders(int km, int kord_tr, int kord_tm, bool with_pt) {
   if (km < 8) return set_error("fast remap needs km >= 8 (got %d)", km);
   if (!(kord_tr > 7 && (kord_tr <= 10 || kord_tr > 16)))
@@ -552,11 +563,9 @@ int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_d
   fa.ncol = ncol; fa.s2 = d.s2; fa.t_min = (float)t_min;
   fa.q_src = q_src; fa.pt_src = pt_src; fa.q_dst = q_dst; fa.pt_dst = pt_dst; fa.cols = cols;
   {
+    if (check_sizes(km, ncol, d.s2)) return 1;
     const char *b = (const char *)cols.d4;
     const long long cut = (long long)sizeof(float4) * km * ncol, span = (const char *)cols.lk - b + 4LL * km * ncol;
-    const long long fld = 4LL * km * d.s2;
-    if (span >= (1LL << 31) || cut >= (1LL << 31) || fld >= (1LL << 31))
-      return set_error("fast remap: a tile's column data or field level set exceeds 2 GiB");
     fa.o_rbt = (unsigned)((const char *)cols.rbt - b);
     fa.o_gg = (unsigned)((const char *)cols.gg - b);
     fa.o_dp1 = (unsigned)((const char *)cols.dp1 - b);
@@ -576,8 +585,9 @@ int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_d
   return 0;
 }
 
-int remap_fast_check(int km, int kord_tr, int kord_tm, bool with_pt) {
-  return check_orders(km, kord_tr, kord_tm, with_pt);
+int remap_fast_check(int km, long long ncol, long long s2, int kord_tr, int kord_tm, bool with_pt) {
+  if (check_orders(km, kord_tr, kord_tm, with_pt)) return 1;
+  return check_sizes(km, ncol, s2);
 }
 
 }  // namespace fv3
