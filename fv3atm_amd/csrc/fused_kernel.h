@@ -99,9 +99,9 @@ __device__ __forceinline__ void q_src(const FusedArgs<T> &a, const StripCtx<T> &
   const int nid = a.nid;
   // EK 1 blocks have every lane inside the compute domain, EK 0 every cell
   const bool inside = EK == 0 || ((EK == 1 || sc.lane_in) && x.row_in);  // EK >= 2: strips
-  if constexpr (EK >= 5) {  // edge strips, lean rows: halo lanes, no corner cell
-    py = (inside ? s.qin : s.qhalo) + x.orow + sc.lo_q;
-    if (a.ntiles > 1 && !inside) {
+  if constexpr (EK >= 5) {  // edge strips, lean rows (inside the domain): halo lanes, no corner cell
+    py = (sc.lane_in ? s.qin : s.qhalo) + x.orow + sc.lo_q;
+    if (a.ntiles > 1 && !sc.lane_in) {
       int ts, si, sj;
       cube_halo_src(blockIdx.z + 1, sc.ic, x.rc, a.ie, ts, si, sj);
       py = s.qsrc0 + (long long)(ts - 1) * a.tq + (long long)(sj - a.jsd) * nid + (si - a.isd);

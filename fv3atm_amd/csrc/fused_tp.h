@@ -241,6 +241,18 @@ __device__ __forceinline__ void edge_coef_side(T qm1, T q0, T qp1, T dmm1, T dm0
   }
 }
 
+// pert_ppm's standard monotone constraint (iv = 1) with selects instead of
+// branches: each outcome is the same expression as pert_ppm1's, so the values
+// are identical; an edge strip evaluates it on every lane of every x-sweep
+__device__ __forceinline__ void pert_ppm1_sel(float &al, float &ar) {
+  const float da1 = al - ar, da2 = da1 * da1, a6da = 3.f * (al + ar) * da1;
+  const bool ext = al * ar < 0.f;
+  const float nl = a6da > da2 ? -2.f * ar : al;
+  const float nr = a6da < -da2 ? -2.f * al : ar;
+  al = ext ? nl : 0.f;
+  ar = ext ? nr : 0.f;
+}
+
 // FP32 edge strips: a lane's cube-edge (bl, br) candidates as fixed linear
 // combinations of its neighbourhood,
 //   l = cl . (qm1, q0, dmm1, dm0, al0, efl),  r = cr . (q0, qp1, dmp1, dm0, alp1, efr),
@@ -382,7 +394,7 @@ __device__ __forceinline__ T xsweep(T q, T c, int i, int is, int ie, int npx, bo
         l += ec.l[4] * al;
         r += ec.r[3] * dm;
       }
-      pert_ppm1(q, l, r, 1);
+      pert_ppm1_sel(l, r);
       if (ec.on) {
         bl = l;
         br = r;
