@@ -30,7 +30,7 @@ HIP_SOURCES = {
     "mapz_plain.hip": ["-ffp-contract=off"],
     "capi.hip": ["-ffp-contract=off"],
     "fused.hip": ["-ffp-contract=off"],
-    "fused32.hip": ["-ffp-contract=fast"],
+    "fused32.hip": ["-ffp-contract=fast"],  # compiled once per part (FUSED32_NPART)
     "mapz_fast.hip": ["-ffp-contract=fast", "-fno-honor-nans"],
     "tlad.hip": ["-ffp-contract=off"],
 }
@@ -45,17 +45,31 @@ def _run(cmd, cwd=None):
     return out
 
 
+# the FP32 fused kernel's launch configurations are spread over this many
+# objects (fv3_internal.h FUSED32_NPART) so that they compile in parallel
+FUSED32_NPART = 6
+
+
 def build_hip():
     os.makedirs(OBJ, exist_ok=True)
-    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    jobs = []  # (source, object, extra flags); the longest compiles first
+    for s in HIP_SOURCES:
+        if not os.path.exists(os.path.join(CSRC, s)):
+            continue
+        if s == "fused32.hip":
+            jobs += [(s, f"fused32_p{p}.o", [f"-DFV3_F32_PART={p}"]) for p in range(FUSED32_NPART)]
+        else:
+            jobs.append((s, s.replace(".hip", ".o"), []))
+    jobs.sort(key=lambda j: j[0] not in ("fused32.hip", "tlad.hip", "fused.hip"))
 
-    def one(s):
-        obj = os.path.join(OBJ, s.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, *HIP_SOURCES[s], "-c", os.path.join(CSRC, s), "-o", obj])
+    def one(job):
+        s, o, extra = job
+        obj = os.path.join(OBJ, o)
+        _run([HIPCC, *HIP_FLAGS, *HIP_SOURCES[s], *extra, "-c", os.path.join(CSRC, s), "-o", obj])
         return obj
 
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(one, srcs))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        objs = list(ex.map(one, jobs))
     out = os.path.join(HERE, "libfv3atm_amd.so")
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", out + ".tmp", *objs])
     os.replace(out + ".tmp", out)

@@ -1,7 +1,16 @@
 // fused32.hip -- FP32 instantiations of the fused transport kernel (built
 // with FMA contraction on: the FP32 form is compared at 1e-5, not bitwise).
+//
+// The launch configurations are instantiated over FUSED32_NPART objects so
+// that they compile in parallel: the build compiles this file once per part
+// with -DFV3_F32_PART=p, and part p instantiates the configurations cfg with
+// cfg % FUSED32_NPART == p.  Part 0 also holds the dispatcher.
 #include "fused.h"
 #include "fused_kernel.h"
+
+#ifndef FV3_F32_PART
+#error "fused32.hip is compiled once per part with -DFV3_F32_PART=<0..FUSED32_NPART-1>"
+#endif
 
 namespace fv3 {
 
@@ -35,31 +44,63 @@ static int go(const FusedArgs<float> &a, const KindStreams &s) {
   return 0;
 }
 
-// W x NT of the lean kind of each case below
+#define FV3_IN_PART(cfg) ((cfg) % FUSED32_NPART == FV3_F32_PART)
+#define FV3_CASE(cfg, ...)                                 \
+  case cfg:                                                \
+    if constexpr (FV3_IN_PART(cfg)) return go<__VA_ARGS__>(a, s); \
+    break;
+
+// the configurations of this part; 1 when cfg belongs to another part
+#define FV3_PART_FN2(p) launch_fused_f32_part##p
+#define FV3_PART_FN(p) FV3_PART_FN2(p)
+int FV3_PART_FN(FV3_F32_PART)(const FusedArgs<float> &a, int cfg, const KindStreams &s) {
+  switch (cfg) {
+    FV3_CASE(0, 4, 2)
+    FV3_CASE(1, 8, 1)
+    FV3_CASE(2, 4, 1)
+    FV3_CASE(3, 2, 2)
+    FV3_CASE(4, 6, 1)
+    FV3_CASE(5, 1, 2)
+    FV3_CASE(6, 1, 1)
+    FV3_CASE(7, 2, 1)
+    FV3_CASE(8, 4, 1, 3)
+    FV3_CASE(9, 4, 2, 3, 4, 1)
+    FV3_CASE(10, 4, 1, 3, 4, 1, 4)
+    FV3_CASE(11, 4, 2, 3, 4, 1, 3)
+    FV3_CASE(12, 8, 2, 3, 4, 1)
+    FV3_CASE(13, 6, 2, 3, 4, 1)
+    FV3_CASE(14, 8, 1, 3, 4, 1)
+    default: break;
+  }
+  return set_error("fused FP32 launch configuration %d is not instantiated in part %d", cfg, FV3_F32_PART);
+}
+#undef FV3_CASE
+
+#if FV3_F32_PART == 0
+static_assert(FUSED32_NPART == 6, "the dispatcher below names every part");
+int launch_fused_f32_part1(const FusedArgs<float> &, int, const KindStreams &);
+int launch_fused_f32_part2(const FusedArgs<float> &, int, const KindStreams &);
+int launch_fused_f32_part3(const FusedArgs<float> &, int, const KindStreams &);
+int launch_fused_f32_part4(const FusedArgs<float> &, int, const KindStreams &);
+int launch_fused_f32_part5(const FusedArgs<float> &, int, const KindStreams &);
+
+// W x NT of the lean kind of each case above
 static const int kRound[] = {8, 8, 4, 4, 6, 2, 1, 2, 4, 8, 4, 8, 16, 12, 8};
 static_assert(sizeof(kRound) / sizeof(kRound[0]) == FUSED32_NCFG, "one entry per launch configuration");
 
 int fused32_round(int cfg) { return cfg >= 0 && cfg < FUSED32_NCFG ? kRound[cfg] : 8; }
 
 int launch_fused_f32(const FusedArgs<float> &a, int cfg, const KindStreams &s) {
-  switch (cfg) {
-    case 0: return go<4, 2>(a, s);
-    case 1: return go<8, 1>(a, s);
-    case 2: return go<4, 1>(a, s);
-    case 3: return go<2, 2>(a, s);
-    case 4: return go<6, 1>(a, s);
-    case 5: return go<1, 2>(a, s);
-    case 6: return go<1, 1>(a, s);
-    case 7: return go<2, 1>(a, s);
-    case 8: return go<4, 1, 3>(a, s);
-    case 9: return go<4, 2, 3, 4, 1>(a, s);
-    case 10: return go<4, 1, 3, 4, 1, 4>(a, s);
-    case 11: return go<4, 2, 3, 4, 1, 3>(a, s);
-    case 12: return go<8, 2, 3, 4, 1>(a, s);
-    case 13: return go<6, 2, 3, 4, 1>(a, s);
-    case 14: return go<8, 1, 3, 4, 1>(a, s);
-    default: return set_error("unknown fused FP32 launch configuration %d", cfg);
+  if (cfg < 0 || cfg >= FUSED32_NCFG) return set_error("unknown fused FP32 launch configuration %d", cfg);
+  switch (cfg % FUSED32_NPART) {
+    case 0: return launch_fused_f32_part0(a, cfg, s);
+    case 1: return launch_fused_f32_part1(a, cfg, s);
+    case 2: return launch_fused_f32_part2(a, cfg, s);
+    case 3: return launch_fused_f32_part3(a, cfg, s);
+    case 4: return launch_fused_f32_part4(a, cfg, s);
+    default: return launch_fused_f32_part5(a, cfg, s);
   }
 }
+#endif
 
 }  // namespace fv3

@@ -50,6 +50,8 @@ struct GridDev {
 // three workgroups per CU for the edge kernels
 constexpr int FUSED32_NCFG = 15;
 constexpr int FUSED32_DEFAULT_CFG = 9;
+// objects the FP32 configurations are compiled into (cfg % FUSED32_NPART)
+constexpr int FUSED32_NPART = 6;
 
 struct Ctx {
   Dims d;
