@@ -17,28 +17,38 @@
  * Parity status.  The reference holds no golden vectors, known-answer tests
  * or fixtures for this path, and its upstream source is absent, so no
  * function can be compared with upstream output.  Each function is pinned
- * instead by what the mathematics fixes (tests/test_oracle_tp.py,
- * tests/test_oracle_mapz.py):
- *   xppm/yppm interior (hord 8-13): exact on linear data (all orders) and on
- *     parabola means (hord 8); unit Courant takes the upwind cell; yppm is
- *     xppm transposed; hord 8/11 obey the discrete max principle.
+ * instead by what the mathematics and the geometry fix (tests/test_oracle_*.py,
+ * tests/test_golden_rational.py):
+ *   xppm/yppm hord 8 (interior and cube edge): exact-rational restatement on
+ *     8-cell lines (tests/golden, hashed fixtures); exact on linear data (all
+ *     orders) and parabola means; the cube-edge face value exact for data
+ *     linear in distance on the real gnomonic spacing; unit Courant takes the
+ *     upwind cell; yppm is xppm transposed; hord 8/11 obey the discrete max
+ *     principle.
  *   pert_ppm iv=1: the constrained parabola is monotone in the cell.
- *   copy_corners: dir 1 and dir 2 agree under the diagonal swap; six-tile
- *     constancy and mass conservation.
+ *   copy_corners: each corner cell copies the cell that continues the sweep's
+ *     grid line around the cube corner on the sphere; dir 1 and dir 2 agree
+ *     under the diagonal swap.
  *   fv_tp_2d, tracer_2d_1L, tracer_2d: constant preservation, six-tile mass
- *     conservation to 1e-13, the nsplit rule, forced q_split.
+ *     conservation to 1e-13, second-order convergence away from the cube
+ *     edges (C24-C96), a full cosine-bell revolution, six-tile rotation
+ *     symmetry, the nsplit rule against hand-computed counts, forced q_split.
  *   deln_flux: zero flux on constants, variance reduction on noise.
- *   cs_profile: C1 spline when unlimited, exact on linear profiles.
+ *   cs_profile kord 9 + map1_ppm: exact-rational restatement on 6- and
+ *     10-layer columns (tests/golden); C1 spline when unlimited, exact on
+ *     linear profiles; 2-delta-p waves flattened, one-layer extrema kept
+ *     curved, smooth monotone profiles unconstrained (kord 9 and 10).
  *   map1_ppm, map_scalar, mapn_tracer, Lagrangian_to_Eulerian subset:
  *     identity remap, column conservation, constancy, refine-then-coarsen,
  *     iv=0 positivity, pressure bookkeeping.
  *   fillz: conservation and non-negativity.
- * Parity unpinned (no check can tell a faithful restatement from a
- * plausible variant): the cube-edge one-sided coefficients of xppm/yppm
- * beyond their linear exactness, the copy_corners orientation relative to
- * upstream beyond its symmetry, the 1 - sin_sg(5) term of the nsplit rule,
- * the hord 9/10/12/13 Huynh-constraint details, and the kord 9/10 extremum
- * branches of cs_profile beyond conservation, positivity and exactness.
+ * Parity unpinned (only the absent upstream source could tell a faithful
+ * restatement from a plausible variant): the constants of the one-sided
+ * cube-edge 3-point face values (11/14, 3/14, 4/7) and of the cs_profile end
+ * closures beyond their linear exactness, the 1 - sin_sg(5) term of the nsplit
+ * rule, the hord 9/10/12/13 Huynh-constraint details beyond linear
+ * exactness, positivity and the max principle, and the kord 10 extremum
+ * branches beyond the wave/extremum rules above.
  */
 #ifndef FV3_ORACLE_H
 #define FV3_ORACLE_H

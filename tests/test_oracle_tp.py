@@ -149,21 +149,6 @@ def test_sphere_mass_conservation_and_constancy():
     assert np.abs(r["q"][NG:-NG, NG:-NG, :, 1, :] - 1.25).max() <= 1e-13
 
 
-def test_nsplit_rule():
-    case = cases.make_case("C16L12jet", n=16, npz=12, nq=1, jet=True, courant=2.4, seed=1)
-    r = oracle.tracer_2d(case, "1L")
-    s5 = case.grids[0].sin_sg[NG:-NG, NG:-NG, 4]
-    expect = []
-    for k in range(case.npz):
-        m = np.maximum(np.abs(case.cx[:16, NG:-NG, k]), np.abs(case.cy[NG:-NG, :16, k]))
-        cm = m.max() if (k + 1) < case.npz // 6 else (m + 1.0 - s5).max()
-        expect.append(int(1.0 + cm))
-    np.testing.assert_array_equal(r["nsplit"], expect)
-    # the winds come back scaled by 1/nsplit(k)
-    for k in range(case.npz):
-        np.testing.assert_allclose(r["cx"][..., k], case.cx[..., k] / expect[k], rtol=1e-15)
-
-
 def test_tracer_2d_forced_split():
     case = cases.make_case("C12L3", n=12, npz=3, nq=1, courant=0.5, q_split=3, seed=3)
     r = oracle.tracer_2d(case, "2d")
