@@ -149,31 +149,106 @@ def lagrangian_to_eulerian_tracers(ctx: Context, km, nq, ak, bk, ptop, delp, q, 
         "Lagrangian_to_Eulerian")
 
 
+def _sizes(ctx: Context, nq: int):
+    """Element counts of the device arrays of a step on ctx (every tile)."""
+    b, nt, npz = ctx.bd, ctx.ntiles, ctx.npz
+    nid, njd = b.ied - b.isd + 1, b.jed - b.jsd + 1
+    nix, niy, nj = b.ie - b.is_ + 2, b.ie - b.is_ + 1, b.je - b.js + 1
+    lev = npz * nt
+    return dict(q=nid * njd * npz * nq * nt, pt=nid * njd * lev, dp=nid * njd * lev,
+                cx=nix * njd * lev, cy=nid * (nj + 1) * lev, mfx=nix * nj * lev,
+                mfy=niy * (nj + 1) * lev, ak=npz + 1)
+
+
+def _dev_args(ctx: Context, nq, spec):
+    """Checks torch device arguments before their addresses cross the C-ABI:
+    on the context's device, contiguous, of the element type the form
+    computes in, and at least as long as the step reads or writes.
+    spec: (name, tensor or None, dtype, size key, required)."""
+    import torch
+    n = _sizes(ctx, nq)
+    dev = None
+    for name, t, dt, key, required in spec:
+        if t is None:
+            if required:
+                raise TypeError(f"{name}: a device tensor is required")
+            continue
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError(f"{name}: a torch tensor on the HIP device is required")
+        if t.dtype != dt:
+            raise TypeError(f"{name}: dtype {t.dtype}, expected {dt}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name}: a contiguous tensor is required")
+        if t.numel() < n[key]:
+            raise ValueError(f"{name}: {t.numel()} elements, the step needs {n[key]}")
+        if dev is None:
+            dev = t.device
+        elif t.device != dev:
+            raise ValueError(f"{name}: on {t.device}, the other arguments on {dev}")
+
+
+def _stream(stream):
+    """The caller's stream, or torch's current stream on the current device."""
+    if stream is not None:
+        return stream
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _step_spec(form, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx, mfy, ak, bk):
+    import torch
+    if form not in (FORM_PLAIN64, FORM_FUSED64, FORM_FUSED32):
+        raise ValueError(f"unknown form {form}")
+    w = torch.float32 if form == FORM_FUSED32 else torch.float64
+    f64 = torch.float64
+    return [("q_in", q_in, w, "q", True), ("q_out", q_out, w, "q", True), ("pt", pt, w, "pt", False),
+            ("dp1", dp1, f64, "dp", True), ("delp_out", delp_out, f64, "dp", True),
+            ("cx", cx, w, "cx", True), ("cy", cy, w, "cy", True), ("mfx", mfx, w, "mfx", True),
+            ("mfy", mfy, w, "mfy", True), ("ak", ak, f64, "ak", True), ("bk", bk, f64, "ak", True)]
+
+
 def step_dev(ctx: Context, form, nq, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx, mfy, ak, bk,
              ptop, hord=8, kord_tr=9, kord_tm=10, t_min=184.0, nsplit_cap=0, stream=None):
-    """Device-resident advection + remap step on torch device tensors."""
+    """Device-resident advection + remap step on torch device tensors (on
+    ``stream``, a hipStream_t handle; default: torch's current stream)."""
+    _dev_args(ctx, nq, _step_spec(form, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx, mfy, ak, bk))
     check(_ffi.lib().fv3_step_dev(ctx.handle, form, nq, ptr(q_in), ptr(q_out), ptr(pt), ptr(dp1),
                                   ptr(delp_out), ptr(cx), ptr(cy), ptr(mfx), ptr(mfy), ptr(ak),
                                   ptr(bk), ptop, hord, kord_tr, kord_tm, t_min, nsplit_cap,
-                                  stream), "fv3_step_dev")
+                                  _stream(stream)), "fv3_step_dev")
+
+
+def _tlad_spec(names_tensors, with_remap):
+    import torch
+    f64 = torch.float64
+    keys = dict(q0="q", dq0="q", q1="q", dq1="q", qbar1="q", qbar0="q", dp1="dp", delp_out="dp",
+                cx="cx", cy="cy", mfx="mfx", mfy="mfy", ak="ak", bk="ak")
+    return [(k, t, f64, keys[k], k != "delp_out" and (with_remap or k not in ("ak", "bk")))
+            for k, t in names_tensors]
 
 
 def step_tl_dev(ctx: Context, per_level, q_split, nq, q0, dq0, q1, dq1, dp1, delp_out, cx, cy, mfx,
                 mfy, ak, bk, ptop, hord=8, kord_tr=9, stream=None):
     """Tangent-linear of the FP64 advection + tracer remap step (frozen
     branches) on torch device tensors: (q0, dq0) -> (q1, dq1)."""
+    _dev_args(ctx, nq, _tlad_spec([("q0", q0), ("dq0", dq0), ("q1", q1), ("dq1", dq1), ("dp1", dp1),
+                                   ("delp_out", delp_out), ("cx", cx), ("cy", cy), ("mfx", mfx),
+                                   ("mfy", mfy), ("ak", ak), ("bk", bk)], kord_tr != 0))
     check(_ffi.lib().fv3_step_tl_dev(ctx.handle, int(per_level), q_split, nq, ptr(q0), ptr(dq0),
                                      ptr(q1), ptr(dq1), ptr(dp1), ptr(delp_out), ptr(cx), ptr(cy),
                                      ptr(mfx), ptr(mfy), ptr(ak), ptr(bk), ptop, hord, kord_tr,
-                                     stream), "fv3_step_tl_dev")
+                                     _stream(stream)), "fv3_step_tl_dev")
 
 
 def step_ad_dev(ctx: Context, per_level, q_split, nq, q0, qbar1, qbar0, dp1, cx, cy, mfx, mfy, ak,
                 bk, ptop, hord=8, kord_tr=9, stream=None):
     """Adjoint of the same step: qbar1 (adjoint of the output) -> qbar0."""
+    _dev_args(ctx, nq, _tlad_spec([("q0", q0), ("qbar1", qbar1), ("qbar0", qbar0), ("dp1", dp1),
+                                   ("cx", cx), ("cy", cy), ("mfx", mfx), ("mfy", mfy), ("ak", ak),
+                                   ("bk", bk)], kord_tr != 0))
     check(_ffi.lib().fv3_step_ad_dev(ctx.handle, int(per_level), q_split, nq, ptr(q0), ptr(qbar1),
                                      ptr(qbar0), ptr(dp1), ptr(cx), ptr(cy), ptr(mfx), ptr(mfy),
-                                     ptr(ak), ptr(bk), ptop, hord, kord_tr, stream),
+                                     ptr(ak), ptr(bk), ptop, hord, kord_tr, _stream(stream)),
           "fv3_step_ad_dev")
 
 

@@ -62,7 +62,21 @@ Dims make_dims(int is, int ie, int js, int je, int isd, int ied, int jsd, int je
   return d;
 }
 
+// A graph-safe context may be referenced by captured graphs: an arena that
+// exists is never freed or moved while graph-safe steps are on (a call that
+// needs it bigger fails instead); one that does not exist yet may be made.
+static bool arena_frozen(const Ctx &c, const void *p, size_t have, size_t want, const char *what) {
+  if (c.graph_safe && p && want > have) {
+    set_error("the context is in graph-safe mode and its %s arena (%zu bytes) cannot grow to %zu "
+              "bytes: captured graphs may hold its address (turn graph-safe mode off first, or "
+              "size the context with a larger eager call before capturing)", what, have, want);
+    return true;
+  }
+  return false;
+}
+
 void *ctx_scratch(Ctx &c, size_t bytes) {
+  if (arena_frozen(c, c.scratch, c.scratch_bytes, bytes, "scratch")) return nullptr;
   if (bytes > c.scratch_bytes) {
     if (c.scratch) {
       (void)hipStreamSynchronize(c.stream);
@@ -82,6 +96,7 @@ void *ctx_scratch(Ctx &c, size_t bytes) {
 }
 
 void *ctx_stage(Ctx &c, size_t bytes) {
+  if (arena_frozen(c, c.stage, c.stage_bytes, bytes, "staging")) return nullptr;
   if (bytes > c.stage_bytes) {
     if (c.stage) {
       (void)hipStreamSynchronize(c.stream);
@@ -683,8 +698,9 @@ int fv3_step_dev(fv3_ctx *h, int form, int nq, const void *q_in, void *q_out, vo
       hipMemcpyAsync(wmx, mfx, bmx, hipMemcpyDeviceToDevice, c.stream);
       hipMemcpyAsync(wmy, mfy, bmy, hipMemcpyDeviceToDevice, c.stream);
       hipMemcpyAsync(delp_out, dp1, bdp, hipMemcpyDeviceToDevice, c.stream);
-      rc = tracer_2d_plain<double>(c, true, (double *)q_out, wdp, wmx, wmy, wcx, wcy, nq, hord, 0, 1.0,
-                                   0, 0, 0.0, 1.0, nullptr);
+      // the context's sub-cycling rule (fv3_ctx_set_transport), as the fused forms
+      rc = tracer_2d_plain<double>(c, c.step_per_level, (double *)q_out, wdp, wmx, wmy, wcx, wcy, nq, hord,
+                                   c.step_per_level ? 0 : c.step_q_split, 1.0, 0, 0, 0.0, 1.0, nullptr);
       if (!rc)
         rc = remap_tracers_plain<double>(c, npz, nq, ak, bk, ptop, delp_out, (double *)q_out,
                                          (double *)pt, kt.data(), kord_tm, t_min, 0);
@@ -828,9 +844,12 @@ int fv3_step_status(fv3_ctx *h, int *nsplit_overflow) {
   Ctx &c = h->c;
   std::lock_guard<std::mutex> lk(c.mu);
   if (!c.step_flag) return set_error("fv3_step_status: graph-safe mode was never enabled");
+  // the step (or a replay of it) may have run on any stream of the device:
+  // wait for all of them before reading its flag
+  FV3_HIP(hipSetDevice(c.dev));
+  FV3_HIP(hipDeviceSynchronize());
   int v = 0;
-  FV3_HIP(hipMemcpyAsync(&v, c.step_flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  FV3_HIP(hipStreamSynchronize(c.stream));
+  FV3_HIP(hipMemcpy(&v, c.step_flag, sizeof(int), hipMemcpyDeviceToHost));
   *nsplit_overflow = v;
   return 0;
 }

@@ -21,10 +21,18 @@ template <typename T>
 int device_nsplit(Ctx &c, const T *cx, const T *cy, bool per_level, int q_split, T *cmax,
                   int *nsplit, T *frac);
 
+// Device scratch handed to a step by its caller instead of the context's
+// arenas (the no-allocation entry point); base == nullptr: the context's.
+struct StepArena {
+  char *base;
+  size_t bytes;
+};
+
 int fused_step(Ctx &c, bool fp32, int nq, const void *q_in, void *q_out, void *pt,
                const double *dp1, double *delp_out, const void *cx, const void *cy,
                const void *mfx, const void *mfy, const double *ak, const double *bk, double ptop,
-               int hord, int kord_tr, int kord_tm, double t_min, int nsplit_cap);
+               int hord, int kord_tr, int kord_tm, double t_min, int nsplit_cap,
+               const StepArena &arena = StepArena{nullptr, 0});
 
 // tangent-linear / adjoint of the FP64 advection + tracer remap step
 int step_tl(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const double *dq0,

@@ -46,14 +46,23 @@ template <typename T>
 static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const double *dp1,
                         double *delp_out, const T *cx, const T *cy, const T *mfx, const T *mfy,
                         const double *ak, const double *bk, double ptop, int hord, int kord_tr,
-                        int kord_tm, double t_min, int nsplit_cap) {
+                        int kord_tm, double t_min, int nsplit_cap, const StepArena &arena) {
   const Dims &d = c.d;
   const int npz = c.npz;
   hipStream_t s = c.stream;
   if (hord < 7 || hord > 13) return set_error("hord=%d: only the monotone orders 7..13 are implemented", hord);
   if (c.graph_safe && nsplit_cap <= 0)
     return set_error("graph-safe steps need nsplit_cap > 0 (no device-to-host read)");
-  // scratch: cmax, frac, nsplit, flag, then q_tmp
+  const bool fast_remap = std::is_same<T, float>::value && c.remap_mode == 0 && npz >= 8;
+  // the per-column remap synchronises the host with the stream, so only the
+  // column-shared FP32 remap can be captured
+  if (c.graph_safe && !fast_remap)
+    return set_error("graph-safe steps need the FP32 form with the column-shared remap and npz >= 8");
+  const bool ext = arena.base != nullptr;
+  if (ext && !fast_remap)
+    return set_error("the no-allocation step needs the FP32 form with the column-shared remap and npz >= 8");
+  // head: cmax, frac, nsplit, flag (the context's staging arena, or the
+  // front of the caller's arena)
   size_t off = 0;
   auto take = [&](size_t b) {
     size_t o = off;
@@ -63,7 +72,17 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   const size_t o_cmax = take(sizeof(T) * npz), o_frac = take(sizeof(T) * npz);
   const size_t o_ns = take(sizeof(int) * npz), o_flag = take(sizeof(int));
   const size_t head = off;
-  char *base = (char *)ctx_stage(c, head);
+  const int nt = c.ntiles;
+  const long long qsize = d.s2 * npz * nq * nt;  // all tiles
+  const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
+  // the FP32 remap's scratch: q_tmp, the pt source copy, every tile's column data
+  const size_t b_q = ((sizeof(T) * qsize) + 255) & ~(size_t)255;
+  const size_t b_pt = pt ? ((sizeof(T) * d.s2 * npz * nt) + 255) & ~(size_t)255 : 0;
+  const size_t b_col = fast_remap ? remap_cols_bytes(npz, (long long)ni * nj) : 0;
+  if (ext && head + b_q + b_pt + b_col * nt > arena.bytes)
+    return set_error("this step needs %zu bytes of scratch; the no-allocation scratch holds %zu "
+                     "(fv3_step_dev allocates its own)", head + b_q + b_pt + b_col * nt, arena.bytes);
+  char *base = ext ? arena.base : (char *)ctx_stage(c, head);
   if (!base) return 1;
   T *cmax = (T *)(base + o_cmax), *frac = (T *)(base + o_frac);
   int *nspl = (int *)(base + o_ns), *flag = c.graph_safe ? c.step_flag : (int *)(base + o_flag);
@@ -77,21 +96,14 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     cap = *std::max_element(h.begin(), h.end());
     if (cap < 1) return set_error("nsplit < 1 from a non-finite Courant number");
   }
-  const int nt = c.ntiles;
-  const long long qsize = d.s2 * npz * nq * nt;  // all tiles
-  const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
   // The FP32 form remaps out of place: the last transport pass of every
   // level lands in q_tmp (the caller's q_out serves as the alternate pass
   // buffer), and the remap writes q_out.
-  const bool fast_remap = std::is_same<T, float>::value && c.remap_mode == 0 && npz >= 8;
   T *q_tmp = nullptr, *pt_src = nullptr;
   std::vector<RemapCols> cols(nt);  // column data of every tile
   if (fast_remap) {
     if (remap_fast_check(npz, (long long)ni * nj, d.s2, kord_tr, kord_tm, pt != nullptr)) return 1;
-    const size_t b_q = ((sizeof(T) * qsize) + 255) & ~(size_t)255;
-    const size_t b_pt = pt ? ((sizeof(T) * d.s2 * npz * nt) + 255) & ~(size_t)255 : 0;
-    const size_t b_col = remap_cols_bytes(npz, (long long)ni * nj);
-    char *sc = (char *)ctx_scratch(c, b_q + b_pt + b_col * nt);
+    char *sc = ext ? arena.base + head : (char *)ctx_scratch(c, b_q + b_pt + b_col * nt);
     if (!sc) return 1;
     q_tmp = (T *)sc;
     pt_src = pt ? (T *)(sc + b_q) : nullptr;
@@ -297,16 +309,17 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
 int fused_step(Ctx &c, bool fp32, int nq, const void *q_in, void *q_out, void *pt,
                const double *dp1, double *delp_out, const void *cx, const void *cy,
                const void *mfx, const void *mfy, const double *ak, const double *bk, double ptop,
-               int hord, int kord_tr, int kord_tm, double t_min, int nsplit_cap) {
+               int hord, int kord_tr, int kord_tm, double t_min, int nsplit_cap,
+               const StepArena &arena) {
   if (fp32)
     return fused_step_t<float>(c, nq, (const float *)q_in, (float *)q_out, (float *)pt, dp1, delp_out,
                                (const float *)cx, (const float *)cy, (const float *)mfx,
                                (const float *)mfy, ak, bk, ptop, hord, kord_tr, kord_tm, t_min,
-                               nsplit_cap);
+                               nsplit_cap, arena);
   return fused_step_t<double>(c, nq, (const double *)q_in, (double *)q_out, (double *)pt, dp1, delp_out,
                               (const double *)cx, (const double *)cy, (const double *)mfx,
                               (const double *)mfy, ak, bk, ptop, hord, kord_tr, kord_tm, t_min,
-                              nsplit_cap);
+                              nsplit_cap, arena);
 }
 
 }  // namespace fv3

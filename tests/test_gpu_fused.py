@@ -217,9 +217,10 @@ def test_step_tracer_2d_rule(gpu_ready, name, q_split):
     adv = oracle.tracer_2d(case, "2d")
     q, delp, pt = oracle.remap_tracers(case, adv["q"], case.dp1, case.pt)
     print(name, "q_split", q_split, "nsplit", np.unique(adv["nsplit"]).tolist(), flush=True)
-    got = run_step(case, fa.FORM_FUSED64, per_level=False, q_split=q_split)
-    assert_close(interior(got["q"]), interior(q), 1e-12, f"{name} tracer_2d q", floor=1e-300)
-    assert_close(interior(got["pt"]), interior(pt), 1e-12, f"{name} tracer_2d pt")
+    for form in (fa.FORM_FUSED64, fa.FORM_PLAIN64):
+        got = run_step(case, form, per_level=False, q_split=q_split)
+        assert_close(interior(got["q"]), interior(q), 1e-12, f"{name} form {form} tracer_2d q", floor=1e-300)
+        assert_close(interior(got["pt"]), interior(pt), 1e-12, f"{name} form {form} tracer_2d pt")
     got32 = run_step(case, fa.FORM_FUSED32, per_level=False, q_split=q_split)
     for iq in range(case.nq):
         e = rel_l2(interior(got32["q"][:, :, :, iq]), interior(q[:, :, :, iq]))

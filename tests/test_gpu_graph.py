@@ -88,3 +88,69 @@ def test_graph_safe_requires_cap(gpu_ready):
                     _dev(case.mfx, f32), _dev(case.mfy, f32), _dev(case.ak, f64), _dev(case.bk, f64),
                     case.ptop, 8, 9, 10, 184.0, 0)
     ctx.close()
+
+
+def _small(case):
+    f32, f64 = torch.float32, torch.float64
+    d = {k: _dev(getattr(case, k), f32) for k in ("cx", "cy", "mfx", "mfy")}
+    d.update(q=_dev(case.q, f32), pt=_dev(case.pt, f32), dp1=_dev(case.dp1, f64),
+             ak=_dev(case.ak, f64), bk=_dev(case.bk, f64))
+    d["q_out"] = torch.zeros_like(d["q"])
+    d["delp"] = torch.zeros_like(d["dp1"])
+    return d
+
+
+def _go(ctx, case, d, form=fa.FORM_FUSED32, nq=None, cap=3):
+    fa.step_dev(ctx, form, nq or case.nq, d["q"], d["q_out"], d["pt"], d["dp1"], d["delp"], d["cx"],
+                d["cy"], d["mfx"], d["mfy"], d["ak"], d["bk"], case.ptop, 8, 9, 10, 184.0, cap)
+
+
+def test_graph_safe_arenas_do_not_move(gpu_ready):
+    # a graph-safe context never frees or moves its scratch: a later call that
+    # would need it bigger fails with a message instead of leaving captured
+    # graphs with a freed address; the FP64 fused form (its remap synchronises
+    # the host) is refused in graph-safe mode
+    small = cases.make_case("C24L8Q2", n=24, npz=8, nq=2, courant=0.5, seed=3, with_pt=True)
+    big = cases.make_case("C24L8Q6", n=24, npz=8, nq=6, courant=0.5, seed=3, with_pt=True)
+    ctx = fa.Context(small.grids, small.npz)
+    ds = _small(small)
+    _go(ctx, small, ds)  # sizes the scratch for two tracers
+    torch.cuda.synchronize()
+    fa.set_graph_safe(ctx, True)
+    _go(ctx, small, ds)
+    torch.cuda.synchronize()
+    with pytest.raises(RuntimeError, match="graph-safe mode"):
+        _go(ctx, big, _small(big))
+    f64 = torch.float64
+    d64 = {k: (v.to(f64) if v.dtype == torch.float32 else v) for k, v in ds.items()}
+    with pytest.raises(RuntimeError, match="graph-safe steps need the FP32 form"):
+        _go(ctx, small, d64, form=fa.FORM_FUSED64)
+    fa.set_graph_safe(ctx, False)
+    _go(ctx, big, _small(big))  # eager again: the scratch may grow
+    torch.cuda.synchronize()
+    ctx.close()
+
+
+def test_python_wrappers_check_device_arguments(gpu_ready):
+    case = cases.make_case("C24L8Q1", n=24, npz=8, nq=1, courant=0.5, seed=4, with_pt=True)
+    ctx = fa.Context(case.grids, case.npz)
+    d = _small(case)
+    bad = dict(d)
+    bad["q"] = d["q"].cpu()
+    with pytest.raises(TypeError, match="q_in"):
+        _go(ctx, case, bad)
+    bad = dict(d)
+    bad["q"] = d["q"].double()
+    with pytest.raises(TypeError, match="dtype"):
+        _go(ctx, case, bad)
+    bad = dict(d)
+    bad["cx"] = d["cx"][:-5]
+    with pytest.raises(ValueError, match="cx"):
+        _go(ctx, case, bad)
+    bad = dict(d)
+    bad["q_out"] = torch.zeros(2 * d["q"].numel(), device="cuda")[::2]
+    with pytest.raises(ValueError, match="contiguous"):
+        _go(ctx, case, bad)
+    _go(ctx, case, d)  # the valid call still runs
+    torch.cuda.synchronize()
+    ctx.close()
