@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 
 from . import _ffi
-from ._ffi import FORM_FUSED32, FORM_FUSED64, FORM_PLAIN64, check, ptr
+from ._ffi import FORM_FUSED32, FORM_FUSED64, FORM_PLAIN64, NOALLOC_SCRATCH_BYTES, check, ptr
 
 NG = 3
 
@@ -216,6 +216,19 @@ def step_dev(ctx: Context, form, nq, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx
                                   ptr(delp_out), ptr(cx), ptr(cy), ptr(mfx), ptr(mfy), ptr(ak),
                                   ptr(bk), ptop, hord, kord_tr, kord_tm, t_min, nsplit_cap,
                                   _stream(stream)), "fv3_step_dev")
+
+
+def step_dev_noalloc(ctx: Context, nq, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx, mfy, ak, bk,
+                     ptop, hord=8, kord_tr=9, kord_tm=10, t_min=184.0, nsplit_cap=0, stream=None):
+    """The FP32 step of step_dev with its scratch in the library's static
+    per-device array (no device allocation; at most NOALLOC_SCRATCH_BYTES)."""
+    _dev_args(ctx, nq, _step_spec(FORM_FUSED32, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx, mfy,
+                                  ak, bk))
+    check(_ffi.lib().fv3_step_dev_noalloc(ctx.handle, FORM_FUSED32, nq, ptr(q_in), ptr(q_out),
+                                          ptr(pt), ptr(dp1), ptr(delp_out), ptr(cx), ptr(cy),
+                                          ptr(mfx), ptr(mfy), ptr(ak), ptr(bk), ptop, hord, kord_tr,
+                                          kord_tm, t_min, nsplit_cap, _stream(stream)),
+          "fv3_step_dev_noalloc")
 
 
 def _tlad_spec(names_tensors, with_remap):

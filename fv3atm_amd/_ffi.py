@@ -18,6 +18,7 @@ IP = ctypes.POINTER(ctypes.c_int)
 DP = ctypes.POINTER(ctypes.c_double)
 
 FORM_PLAIN64, FORM_FUSED64, FORM_FUSED32 = 0, 1, 2
+NOALLOC_SCRATCH_BYTES = 256 << 20  # FV3_NOALLOC_SCRATCH_BYTES
 
 
 class Bounds(ctypes.Structure):
@@ -46,6 +47,7 @@ SIGNATURES = {
     "fv3_cs_profile": (I, [P, P, P, I, I, I, I, I]),
     "fv3_lagrangian_to_eulerian_tracers": (I, [P, I, I, P, P, D, P, P, P, P, I, D, I]),
     "fv3_step_dev": (I, [P, I, I, P, P, P, P, P, P, P, P, P, P, P, D, I, I, I, D, I, P]),
+    "fv3_step_dev_noalloc": (I, [P, I, I, P, P, P, P, P, P, P, P, P, P, P, D, I, I, I, D, I, P]),
     "fv3_ctx_set_timing": (I, [P, I]),
     "fv3_step_timing": (I, [P, DP, DP, IP]),
     "fv3_ctx_set_launch": (I, [P, I, I, I]),

@@ -33,6 +33,7 @@ HIP_SOURCES = {
     "fused32.hip": ["-ffp-contract=fast"],  # compiled once per part (FUSED32_NPART)
     "mapz_fast.hip": ["-ffp-contract=fast", "-fno-honor-nans"],
     "tlad.hip": ["-ffp-contract=off"],
+    "noalloc.hip": ["-ffp-contract=off"],
 }
 
 

@@ -13,10 +13,6 @@
 #include "fv3_internal.h"
 #include "mapz_column.h"
 
-struct fv3_ctx {
-  fv3::Ctx c;
-};
-
 namespace fv3 {
 
 static thread_local char g_err[1024] = "";
@@ -189,6 +185,10 @@ int fv3_ctx_create(const fv3_bounds *bd, int npx, int npy, int npz, int ntiles, 
   c.npz = npz;
   c.ntiles = ntiles;
   (void)hipGetDevice(&c.dev);
+  if (noalloc_setup(c.dev)) {
+    delete h;
+    return 1;
+  }
   c.d = make_dims(bd->is, bd->ie, bd->js, bd->je, bd->isd, bd->ied, bd->jsd, bd->jed, npx, npy, 15u,
                   false, 0);
   if (hipStreamCreate(&c.stream) != hipSuccess) {

@@ -120,6 +120,10 @@ int set_error(const char *fmt, ...);
 void *ctx_scratch(Ctx &c, size_t bytes);
 void *ctx_stage(Ctx &c, size_t bytes);
 
+// the no-allocation step's device scratch: looks up its address on device
+// dev once (the first context created on that device does it)
+int noalloc_setup(int dev);
+
 template <typename T>
 const GridDev<T> &grid_of(const Ctx &c);
 
@@ -154,3 +158,8 @@ int cs_profile_rows(Ctx &c, const double *a1, const double *dp, const double *qs
                     int im, int km, int iv, int kord);
 
 }  // namespace fv3
+
+// the opaque handle of the C-ABI (include/fv3atm_amd.h)
+struct fv3_ctx {
+  fv3::Ctx c;
+};

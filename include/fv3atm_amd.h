@@ -15,6 +15,8 @@
 #ifndef FV3ATM_AMD_H
 #define FV3ATM_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -174,6 +176,26 @@ int fv3_step_timing(fv3_ctx *ctx, double *transport_ms, double *remap_ms, int *p
  * results are not valid). */
 int fv3_ctx_set_graph_safe(fv3_ctx *ctx, int on);
 int fv3_step_status(fv3_ctx *ctx, int *nsplit_overflow);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same step as fv3_step_dev in the FP32 form
+ * (form must be FV3_FORM_FUSED32, the context must use the column-shared
+ * remap, npz >= 8), with every scratch buffer (Courant maxima, sub-cycle
+ * counts, the alternate pass buffer, the pt source copy, the remap's column
+ * data) carved from one file-scope device array of FV3_NOALLOC_SCRATCH_BYTES
+ * per device instead of the context's arenas; it calls no hipMalloc,
+ * hipFree or allocating library routine.  A step that needs more scratch
+ * fails with a message naming the size.  The array is shared by every call
+ * on a device and keeps the last call's contents (each step initialises what
+ * it reads); calls that overlap on one device race.  Results are those of
+ * fv3_step_dev. */
+#define FV3_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int fv3_step_dev_noalloc(fv3_ctx *ctx, int form, int nq, const void *q_in, void *q_out, void *pt,
+                         const double *dp1, double *delp_out, const void *cx, const void *cy,
+                         const void *mfx, const void *mfy, const double *ak, const double *bk,
+                         double ptop, int hord, int kord_tr, int kord_tm, double t_min,
+                         int nsplit_cap, void *stream);
 
 /* Tangent-linear and adjoint of the FP64 advection + tracer remap step: the
  * derivatives of tracer_2d (per_level = 0; q_split > 0 forces nsplit) or
