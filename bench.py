@@ -7,8 +7,9 @@ profile) followed by the tracer (kord 9) and pt (kord 10) remap of
 Lagrangian_to_Eulerian, with every input already resident in HBM.  The
 timed form is the fused FP32-bulk form (FP64 thickness/pressure); the plain
 FP64 form is timed beside it, and the FP32 result is compared with the FP64
-one.  The CPU baseline is the FP64 CPU oracle on a bounded sample of the same
-workload (one tracer, every level, the full tile) on one host core.
+one.  The CPU baseline is the FP64 CPU oracle (OpenMP) on a bounded sample of the
+same workload (two tracers + pt, every level, the full tile) on all host
+threads, with a one-thread figure beside it.
 
 Prints ONE JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -157,23 +158,58 @@ def profiled_valu(n, npz, nq, cfg, rows):
     return profiled("bench-valu", "valu_per_pass", n, npz, nq, cfg, rows)
 
 
-def cpu_baseline(case, nq=2):
-    """FP64 oracle on the full tile, every level, nq tracers (+ pt remap)."""
-    from oracle import oracle
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, nq_all=2, nq_one=1, reps_all=5, reps_one=3):
+    """FP64 oracle (C, OpenMP over levels in the transport and rows in the
+    remap) on the full tile, every level, with pt: all host threads on nq_all
+    tracers (median of reps_all steps after one warm-up), and one thread on
+    nq_one tracers (median of reps_one after one warm-up)."""
     import copy
-    c1 = copy.copy(case)
-    c1.nq = nq
+    import ctypes
+    from oracle import oracle
+    L = oracle.lib()
+    L.fv3o_set_threads.argtypes = [ctypes.c_int]
+    L.fv3o_set_threads.restype = ctypes.c_int
     bases = case.meta["q_bases"][0]
-    c1.q = np.asfortranarray(np.stack([bases[i % len(bases)][:, :, None] * case.meta["lev"][None, None, :]
-                                       for i in range(nq)], axis=-1))
-    t0 = time.perf_counter()
-    adv = oracle.tracer_2d(c1, "1L")
-    oracle.remap_tracers(c1, adv["q"], c1.dp1, c1.pt)
-    t = time.perf_counter() - t0
-    cells = case.n * case.n * case.npz * nq
-    return dict(value=cells / t / 1e9, unit="Gcell-tracer updates/s", cores=1, kind="port",
-                sample=f"FP64 oracle (C restatement), full C{case.n} tile, {case.npz} levels, "
-                       f"{nq} tracers (+pt remap), one step in {t:.1f} s on one host core")
+
+    def timed(nq, reps):
+        c1 = copy.copy(case)
+        c1.nq = nq
+        c1.q = np.asfortranarray(np.stack([bases[i % len(bases)][:, :, None] * case.meta["lev"][None, None, :]
+                                           for i in range(nq)], axis=-1))
+        ts = []
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            adv = oracle.tracer_2d(c1, "1L")
+            oracle.remap_tracers(c1, adv["q"], c1.dp1, c1.pt)
+            if r:  # the first step is the warm-up
+                ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), case.n * case.n * case.npz * nq
+
+    omp_env = os.environ.get("OMP_NUM_THREADS")
+    nthreads = L.fv3o_set_threads(int(omp_env) if omp_env and omp_env.isdigit() else os.cpu_count())
+    t_all, c_all = timed(nq_all, reps_all)
+    L.fv3o_set_threads(1)
+    t_one, c_one = timed(nq_one, reps_one)
+    L.fv3o_set_threads(nthreads)
+    return dict(value=c_all / t_all / 1e9, unit="Gcell-tracer updates/s", cores=nthreads, kind="port",
+                sample=f"FP64 oracle (C restatement, OpenMP over levels and rows), full C{case.n} tile, "
+                       f"{case.npz} levels, {nq_all} tracers + pt remap; median of {reps_all} steps after "
+                       f"one warm-up, {t_all:.2f} s per step on {nthreads} threads",
+                one_core=dict(value=c_one / t_one / 1e9, cores=1,
+                              sample=f"{nq_one} tracer + pt remap, median of {reps_one} steps after one "
+                                     f"warm-up, {t_one:.2f} s per step"),
+                nproc=os.cpu_count(), cpu_model=_cpu_model(), omp_num_threads=omp_env)
 
 
 def run_sphere(args, torch, fa, cases):
@@ -400,6 +436,10 @@ def main():
         ms64, _, _ = run_form(fa, ctx64, fa.FORM_PLAIN64, d64, case, args.plain_steps, 0, cap, torch)
         log(f"plain FP64: {ms64:.2f} ms/step")
         out["plain64_ms_per_step"] = round(ms64, 3)
+        # the fused FP64 form (the FP32 form's kernels in FP64, per-column remap)
+        ms64f, _, _ = run_form(fa, ctx64, fa.FORM_FUSED64, d64, case, args.plain_steps, 1, cap, torch)
+        log(f"fused FP64: {ms64f:.2f} ms/step")
+        out["fused64_ms_per_step"] = round(ms64f, 3)
         del d64
         ctx64.close()
         torch.cuda.empty_cache()
@@ -518,6 +558,7 @@ def main():
     out_line.update(out)
     if "plain64_ms_per_step" in out:
         out_line["speedup_vs_plain64"] = round(out["plain64_ms_per_step"] / ms, 3)
+        out_line["speedup_vs_fused64"] = round(out["fused64_ms_per_step"] / ms, 3)
     ctx.close()
     del d
     torch.cuda.empty_cache()

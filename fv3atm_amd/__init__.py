@@ -15,7 +15,8 @@ import ctypes
 import numpy as np
 
 from . import _ffi
-from ._ffi import FORM_FUSED32, FORM_FUSED64, FORM_PLAIN64, NOALLOC_SCRATCH_BYTES, check, ptr
+from ._ffi import (AGRID, CGRID, DGRID, FORM_FUSED32, FORM_FUSED64, FORM_PLAIN64,
+                   NOALLOC_SCRATCH_BYTES, check, ptr)
 
 NG = 3
 
@@ -263,6 +264,81 @@ def step_ad_dev(ctx: Context, per_level, q_split, nq, q0, qbar1, qbar0, dp1, cx,
                                      ptr(qbar0), ptr(dp1), ptr(cx), ptr(cy), ptr(mfx), ptr(mfy),
                                      ptr(ak), ptr(bk), ptop, hord, kord_tr, _stream(stream)),
           "fv3_step_ad_dev")
+
+
+def halo_update(ctx: Context, field, nslab, stream=None):
+    """mpp_update_domains of a scalar on a six-tile context: the 3-cell edge
+    halos of nslab (isd:ied, jsd:jed) slabs per tile (tiles outermost) from
+    the neighbouring faces, on the device."""
+    import torch
+    if not (isinstance(field, torch.Tensor) and field.is_cuda and field.is_contiguous()):
+        raise TypeError("field: a contiguous torch tensor on the HIP device is required")
+    if field.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"field: float32 or float64 required, got {field.dtype}")
+    b = ctx.bd
+    need = 6 * nslab * (b.ied - b.isd + 1) * (b.jed - b.jsd + 1)
+    if ctx.ntiles != 6 or field.numel() < need:
+        raise ValueError(f"field: a six-tile context and {need} elements are required")
+    check(_ffi.lib().fv3_halo_update(ctx.handle, ptr(field), nslab, field.element_size(),
+                                     _stream(stream)), "fv3_halo_update")
+
+
+def halo_update_vector(ctx: Context, u, v, nslab, grid, stream=None):
+    """mpp_update_domains of a vector pair (grid AGRID, CGRID or DGRID: the
+    staggering of u and v) on a six-tile context, components turned into
+    each face's index directions across rotated cube edges."""
+    import torch
+    b = ctx.bd
+    nid, njd = b.ied - b.isd + 1, b.jed - b.jsd + 1
+    shapes = {_ffi.AGRID: (nid * njd, nid * njd), _ffi.CGRID: ((nid + 1) * njd, nid * (njd + 1)),
+              _ffi.DGRID: (nid * (njd + 1), (nid + 1) * njd)}
+    if grid not in shapes:
+        raise ValueError(f"grid must be AGRID, CGRID or DGRID, got {grid}")
+    for name, t, per in (("u", u, shapes[grid][0]), ("v", v, shapes[grid][1])):
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()):
+            raise TypeError(f"{name}: a contiguous torch tensor on the HIP device is required")
+        if t.dtype != u.dtype or t.dtype not in (torch.float32, torch.float64):
+            raise TypeError(f"{name}: float32 or float64, the same for u and v")
+        if ctx.ntiles != 6 or t.numel() < 6 * nslab * per:
+            raise ValueError(f"{name}: a six-tile context and {6 * nslab * per} elements are required")
+    check(_ffi.lib().fv3_halo_update_vector(ctx.handle, ptr(u), ptr(v), nslab, grid, u.element_size(),
+                                            _stream(stream)), "fv3_halo_update_vector")
+
+
+def lagrangian_to_eulerian_dev(ctx: Context, form, km, nq, ak, bk, ptop, state, kord_tr=9, kord_tm=10,
+                               kord_mt=9, kord_wz=9, hydrostatic=False, consv=0.0, t_min=184.0,
+                               fill=False, akap=287.05 / 1004.6, cp=1004.6, rdgas=287.05, grav=9.80665,
+                               p_ref=1.0e5, mdt=0.0, last_step=False, fast_physics=None, stream=None):
+    """fv_mapz.F90 Lagrangian_to_Eulerian on device tensors.  ``state`` maps
+    delp, u, v, pt, pe, peln, pk, pkz, ps (and q, w, delz, ws, te0_2d when
+    used) to contiguous device tensors in the layouts of
+    fv3_lagrangian_to_eulerian_dev; field tensors are float64 for
+    FORM_PLAIN64 and float32 for FORM_FUSED32, pressures and energies
+    float64.  fast_physics (a Python callable taking the fv3_l2e_state) runs
+    after the pressures and before the energy fix-up.  Returns dtmp."""
+    import torch
+    f = torch.float64 if form == FORM_PLAIN64 else torch.float32
+    dbl = ("delp", "pe", "peln", "pk", "pkz", "ps", "te0_2d")
+    for k, t in state.items():
+        want = torch.float64 if k in dbl else f
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous() and t.dtype == want):
+            raise TypeError(f"{k}: a contiguous {want} tensor on the HIP device is required")
+    kt = np.full(max(nq, 1), kord_tr, dtype=np.int32)
+    dtmp = ctypes.c_double(0.0)
+    hook = _ffi.FAST_PHYSICS_FN(lambda user, st: int(bool(fast_physics(st.contents)))) if fast_physics \
+        else _ffi.FAST_PHYSICS_FN()
+    g = lambda k: ptr(state.get(k))
+    a = _ffi.L2EArgs(km=km, nq=nq, ak=ptr(ak), bk=ptr(bk), ptop=ptop, akap=akap, cp=cp, rdgas=rdgas,
+                     grav=grav, consv=consv, t_min=t_min, p_ref=p_ref, mdt=mdt,
+                     hydrostatic=int(hydrostatic), fill=int(fill), kord_mt=kord_mt, kord_wz=kord_wz,
+                     kord_tm=kord_tm, last_step=int(last_step), kord_tr=ptr(kt), delp=g("delp"),
+                     u=g("u"), v=g("v"), w=g("w"), delz=g("delz"), pt=g("pt"), q=g("q"), ws=g("ws"),
+                     pe=g("pe"), peln=g("peln"), pk=g("pk"), pkz=g("pkz"), ps=g("ps"),
+                     te0_2d=g("te0_2d"), dtmp_out=ctypes.cast(ctypes.pointer(dtmp), ctypes.c_void_p),
+                     fast_physics=hook, fast_physics_user=None)
+    check(_ffi.lib().fv3_lagrangian_to_eulerian_dev(ctx.handle, form, ctypes.byref(a), _stream(stream)),
+          "Lagrangian_to_Eulerian")
+    return dtmp.value
 
 
 def set_graph_safe(ctx: Context, on: bool = True):

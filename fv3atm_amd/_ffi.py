@@ -19,6 +19,7 @@ DP = ctypes.POINTER(ctypes.c_double)
 
 FORM_PLAIN64, FORM_FUSED64, FORM_FUSED32 = 0, 1, 2
 NOALLOC_SCRATCH_BYTES = 256 << 20  # FV3_NOALLOC_SCRATCH_BYTES
+AGRID, CGRID, DGRID = 1, 2, 3
 
 
 class Bounds(ctypes.Structure):
@@ -30,6 +31,24 @@ class Grid(ctypes.Structure):
                                  "del6_u", "del6_v")] + [("da_min", D)] + \
                [(k, I) for k in ("sw_corner", "se_corner", "nw_corner", "ne_corner",
                                  "bounded_domain", "grid_type")]
+
+
+class L2EState(ctypes.Structure):
+    _fields_ = [(k, I) for k in ("npz", "nq", "ntiles", "kmp", "last_step")] + [("mdt", D)] + \
+               [(k, P) for k in ("delp", "pe", "peln", "pk", "pkz", "ps", "te0_2d", "pt", "q", "delz",
+                                 "w", "stream")]
+
+
+FAST_PHYSICS_FN = ctypes.CFUNCTYPE(ctypes.c_int, P, ctypes.POINTER(L2EState))
+
+
+class L2EArgs(ctypes.Structure):
+    _fields_ = [("km", I), ("nq", I), ("ak", P), ("bk", P)] + \
+               [(k, D) for k in ("ptop", "akap", "cp", "rdgas", "grav", "consv", "t_min", "p_ref", "mdt")] + \
+               [(k, I) for k in ("hydrostatic", "fill", "kord_mt", "kord_wz", "kord_tm", "last_step")] + \
+               [(k, P) for k in ("kord_tr", "delp", "u", "v", "w", "delz", "pt", "q", "ws", "pe", "peln",
+                                 "pk", "pkz", "ps", "te0_2d", "dtmp_out")] + \
+               [("fast_physics", FAST_PHYSICS_FN), ("fast_physics_user", P)]
 
 
 # every symbol the header declares, with its ctypes signature
@@ -56,6 +75,10 @@ SIGNATURES = {
     "fv3_ctx_set_transport": (I, [P, I, I]),
     "fv3_ctx_set_graph_safe": (I, [P, I]),
     "fv3_step_status": (I, [P, IP]),
+    "fv3_halo_update": (I, [P, P, I, I, P]),
+    "fv3_halo_update_vector": (I, [P, P, P, I, I, I, P]),
+    "fv3_lagrangian_to_eulerian_dev": (I, [P, I, ctypes.POINTER(L2EArgs), P]),
+    "fv3_kmp": (I, [I, D, P, P, IP]),
     "fv3_step_tl_dev": (I, [P, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, D, I, I, P]),
     "fv3_step_ad_dev": (I, [P, I, I, I, P, P, P, P, P, P, P, P, P, P, D, I, I, P]),
 }

@@ -34,6 +34,8 @@ HIP_SOURCES = {
     "mapz_fast.hip": ["-ffp-contract=fast", "-fno-honor-nans"],
     "tlad.hip": ["-ffp-contract=off"],
     "noalloc.hip": ["-ffp-contract=off"],
+    "halo.hip": ["-ffp-contract=off"],
+    "l2e.hip": ["-ffp-contract=off"],
 }
 
 

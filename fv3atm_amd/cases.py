@@ -76,6 +76,7 @@ class Grid:
     ne_corner: bool = True
     bounded_domain: bool = False
     grid_type: int = 0
+    xyz_n: np.ndarray = None  # corner unit vectors (3, isd:ied+1, jsd:jed+1), halo-folded
 
     @property
     def npx(self):
@@ -92,10 +93,10 @@ def make_grid(n: int, tile: int = 1, radius: float = RADIUS) -> Grid:
     z = lambda *s: np.zeros(s, dtype=np.float64, order="F")
     g = dict(area=z(nd, nd), rarea=z(nd, nd), dx=z(nd, nd + 1), dy=z(nd + 1, nd),
              dxa=z(nd, nd), dya=z(nd, nd), sin_sg=z(nd, nd, 9), del6_u=z(nd, nd + 1),
-             del6_v=z(nd + 1, nd), xyz_c=z(3, nd, nd))
+             del6_v=z(nd + 1, nd), xyz_c=z(3, nd, nd), xyz_n=z(3, nd + 1, nd + 1))
     rc = lib.fv3c_grid(n, tile, radius, _p(g["area"]), _p(g["rarea"]), _p(g["dx"]), _p(g["dy"]),
                        _p(g["dxa"]), _p(g["dya"]), _p(g["sin_sg"]), _p(g["del6_u"]),
-                       _p(g["del6_v"]), _p(g["xyz_c"]), None)
+                       _p(g["del6_v"]), _p(g["xyz_c"]), _p(g["xyz_n"]))
     if rc:
         raise ValueError(f"fv3c_grid failed for n={n} tile={tile}")
     return Grid(n=n, tile=tile, **g)
@@ -306,3 +307,68 @@ def config(idx: int, **over) -> Case:
         raise ValueError(idx)
     kw.update(over)
     return make_case(**kw)
+
+
+RDGAS, CP_AIR, GRAV = 287.05, 1004.6, 9.80665
+KAPPA = RDGAS / CP_AIR
+
+
+def l2e_state(case: Case, seed: int = 0, hydrostatic: bool = False) -> dict:
+    """The rest of the dynamical state that Lagrangian_to_Eulerian remaps, on
+    the case's grid and Lagrangian thickness dp1: D-grid winds u (isd:ied,
+    jsd:jed+1, km) and v (isd:ied+1, jsd:jed, km) from a sheared solid-body
+    flow plus seeded waves, vertical velocity w (isd:ied, jsd:jed, km) and its
+    surface value ws (is:ie, js:je), hydrostatic layer depths delz (is:ie,
+    js:je, km) and pkz (is:ie, js:je, km), pt, and the tracers.  Arrays carry
+    the tile as the last axis for six-tile cases."""
+    n, km = case.n, case.npz
+    nd = n + 2 * NG
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(0.5, 1.5, 6)
+    tiles = range(case.ntiles)
+    out = {k: [] for k in ("u", "v", "w", "ws", "delz", "pkz")}
+    for t in tiles:
+        g = case.grids[t]
+        dp = case.dp1[..., t] if case.ntiles > 1 else case.dp1
+        pe = PTOP + np.concatenate([np.zeros((nd, nd, 1)), np.cumsum(dp, axis=2)], axis=2)
+        lev = np.arange(km)[None, None, :] / max(km - 1, 1)
+        # winds: functions of the face-point positions (halo-folded), so the
+        # halos are the neighbours' values in this face's directions only
+        # approximately; Lagrangian_to_Eulerian reads the compute domain
+        X = g.xyz_n
+        yf = 0.5 * (X[:, 1:, :] + X[:, :-1, :])   # y-faces (nd, nd+1)
+        xf = 0.5 * (X[:, :, 1:] + X[:, :, :-1])   # x-faces (nd+1, nd)
+        su = 10.0 + 30.0 * np.sin(np.pi * lev)
+        u = (amp[0] * np.sin(2.0 * yf[1] + 0.4) + amp[1] * yf[2])[..., None] * su
+        v = (amp[2] * np.cos(1.5 * xf[0] - 0.2) + amp[3] * xf[1] * xf[2])[..., None] * su
+        c = g.xyz_c
+        w = (amp[4] * np.sin(3.0 * c[0] + 1.0) * np.cos(2.0 * c[1]))[..., None] * (0.2 + np.sin(np.pi * lev))
+        ws = 0.05 * amp[5] * np.sin(2.0 * c[2] + 0.3)[NG:-NG, NG:-NG]
+        # hydrostatic thickness and p^kappa layer means of the Lagrangian layers
+        T = 300.0 - 70.0 * lev * 0 + 0.0 * dp  # placeholder replaced below
+        lp = np.log(pe)
+        pk = np.exp(KAPPA * lp)
+        pkz = (pk[..., 1:] - pk[..., :-1]) / (KAPPA * (lp[..., 1:] - lp[..., :-1]))
+        ptv = case.pt[..., t] if (case.pt is not None and case.ntiles > 1) else case.pt
+        if ptv is None:
+            ptv = 184.0 + 116.0 * ((pe[..., 1:] + pe[..., :-1]) / (2 * PS)) ** KAPPA
+        T = ptv * pkz
+        delz = -RDGAS / GRAV * T * (lp[..., 1:] - lp[..., :-1])
+        out["u"].append(np.asfortranarray(u))
+        out["v"].append(np.asfortranarray(v))
+        out["w"].append(np.asfortranarray(w))
+        out["ws"].append(np.asfortranarray(ws))
+        out["delz"].append(np.asfortranarray(delz[NG:-NG, NG:-NG]))
+        out["pkz"].append(np.asfortranarray(pkz[NG:-NG, NG:-NG]))
+    st = {k: (np.stack(v, axis=-1) if case.ntiles > 1 else v[0]) for k, v in out.items()}
+    st = {k: np.asfortranarray(v) for k, v in st.items()}
+    st["delp"] = np.asfortranarray(case.dp1.copy())
+    pt = case.pt
+    if pt is None:
+        lev_pe = PTOP + np.cumsum(case.dp1, axis=2) - 0.5 * case.dp1
+        pt = 184.0 + 116.0 * (lev_pe / PS) ** KAPPA
+    st["pt"] = np.asfortranarray(pt.copy())
+    st["q"] = np.asfortranarray(case.q.copy())
+    if hydrostatic:
+        del st["w"], st["ws"], st["delz"]
+    return st

@@ -270,6 +270,8 @@ int fv3_ctx_destroy(fv3_ctx *h) {
   (void)hipFree(c.scratch);
   (void)hipFree(c.stage);
   (void)hipFree(c.step_flag);
+  for (auto *t : c.halo_tab) (void)hipFree(t);
+  (void)hipFree(c.l2e_buf);
   for (auto &e : c.ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &a : c.aux)

@@ -106,6 +106,13 @@ struct Ctx {
   hipEvent_t chunk_ev = nullptr;
   int last_passes = 0;
   bool ev_valid = false;
+  // six-face halo tables of the halo service (scalar, A, C, D grids), built
+  // on first use
+  void *halo_tab[4] = {nullptr, nullptr, nullptr, nullptr};
+  int halo_tab_n[4] = {0, 0, 0, 0};
+  // Lagrangian_to_Eulerian buffers that outlive the column-kernel calls
+  void *l2e_buf = nullptr;
+  size_t l2e_bytes = 0;
 };
 
 // error reporting (thread-local last error string)

@@ -108,6 +108,7 @@ template int remap_tracers_plain<double>(Ctx &, int, int, const double *, const 
 template int remap_tracers_plain<float>(Ctx &, int, int, const double *, const double *, double,
                                         double *, float *, float *, const int *, int, double, int);
 template int remap_launch<double>(Ctx &, RemapArgs<double> &);
+template int remap_launch<float>(Ctx &, RemapArgs<float> &);
 
 __global__ void __launch_bounds__(64) k_cs_rows(const double *a1, const double *dp, const double *qs,
                                                 double *out, int im, int km, int iv, int kord,

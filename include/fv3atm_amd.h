@@ -197,6 +197,83 @@ int fv3_step_dev_noalloc(fv3_ctx *ctx, int form, int nq, const void *q_in, void 
                          double ptop, int hord, int kord_tr, int kord_tm, double t_min,
                          int nsplit_cap, void *stream);
 
+/* Halo service of a six-tile context: the one-device equivalent of FMS
+ * mpp_update_domains on the cubed sphere (upstream fv_dynamics calls it for
+ * the prognostic fields; moving_nest/fv_moving_nest.F90:355-387 shows their
+ * range).  Device arrays, nslab stacked slabs per tile, tiles outermost;
+ * elem_bytes 4 (float) or 8 (double).  The 3-cell edge halos are filled from
+ * the neighbouring faces; the 3x3 corner blocks, which lie on no face, are
+ * left as they are (copy_corners fills them per sweep).  stream NULL: the
+ * context's stream.
+ * fv3_halo_update: a scalar at cell centres, slabs (isd:ied, jsd:jed).
+ * fv3_halo_update_vector: a vector pair whose components are along each
+ * face's +i and +j index directions, turned into the receiving face's
+ * directions where the neighbour face is rotated (FMS's vector updates):
+ *   FV3_AGRID  u, v at cell centres, (isd:ied, jsd:jed) each;
+ *   FV3_CGRID  u on x-faces (isd:ied+1, jsd:jed), v on y-faces (isd:ied, jsd:jed+1);
+ *   FV3_DGRID  u on y-faces (isd:ied, jsd:jed+1), v on x-faces (isd:ied+1, jsd:jed)
+ *              (the dycore's prognostic winds). */
+enum { FV3_AGRID = 1, FV3_CGRID = 2, FV3_DGRID = 3 };
+int fv3_halo_update(fv3_ctx *ctx, void *field, int nslab, int elem_bytes, void *stream);
+int fv3_halo_update_vector(fv3_ctx *ctx, void *u, void *v, int nslab, int grid, int elem_bytes,
+                           void *stream);
+
+/* The full Lagrangian_to_Eulerian of fv_mapz.F90 on device arrays (no
+ * single upstream call has this argument list: upstream passes ~60 scalars
+ * and arrays; this block carries the ones the remap reads and writes).
+ * Element type of the field arrays (u, v, w, delz, pt, q, ws): double for
+ * form FV3_FORM_PLAIN64, float for FV3_FORM_FUSED32; pressures, delp and
+ * energies are double.  Layouts per tile, tiles outermost:
+ *   delp, w, pt (isd:ied, jsd:jed, km); q (isd:ied, jsd:jed, km, nq);
+ *   u (isd:ied, jsd:jed+1, km) and v (isd:ied+1, jsd:jed, km): D-grid winds;
+ *   delz, pkz (is:ie, js:je, km); ws, te0_2d (is:ie, js:je); ps (isd:ied, jsd:jed);
+ *   pe (is-1:ie+1, km+1, js-1:je+1); peln (is:ie, km+1, js:je); pk (is:ie, js:je, km+1).
+ * delp holds the Lagrangian thickness on the compute domain and its one-cell
+ * ring (the winds' interfaces average neighbouring columns; refresh a six-tile
+ * delp with fv3_halo_update first); it returns the Eulerian thickness.
+ * Remapped: u, v (kord_mt, iv = -1), w (kord_wz, iv = -2 with the surface
+ * value ws) and delz (as -delz/delp, kord_wz) when hydrostatic == 0, the
+ * tracers (kord_tr) and pt (kord_tm, floor t_min) as
+ * fv3_lagrangian_to_eulerian_tracers.  Recomputed on the Eulerian levels: pe
+ * (the ring keeps the Lagrangian interfaces), ps, peln = log pe, pk =
+ * pe^akap, and pkz (hydrostatic: the layer mean (pk(k+1)-pk(k))/(akap
+ * (peln(k+1)-peln(k))); otherwise (-rdgas/grav delp/delz pt)^(akap/(1-akap)), the p^akap
+ * of p = rho R pt p^akap).  With
+ * te0_2d != NULL it receives the column total energy before the remap,
+ * sum_k (cp pt pkz + (ua^2 + va^2)/2) dp (pkz as passed in, ua/va the face
+ * averages of u/v); with consv > 0 the domain-wide loss of that energy is
+ * added back as pt += dtmp/pkz, dtmp = consv (E0 - E1) / sum cp (ps-ptop)
+ * area, from compensated FP64 sums (dtmp_out, a host pointer or NULL).
+ * fast_physics (may be NULL) is called once after the pressures and before
+ * the energy fix-up, with the Eulerian state (the GFDL_interstitial fields,
+ * ccpp/data/CCPP_typedefs.F90:434-504; kmp by the pfull rule of :1706-1718
+ * at p_ref); a nonzero return fails the call. */
+typedef struct {
+  int npz, nq, ntiles, kmp, last_step;
+  double mdt;
+  double *delp, *pe, *peln, *pk, *pkz, *ps, *te0_2d;
+  void *pt, *q, *delz, *w;
+  void *stream;
+} fv3_l2e_state;
+typedef int (*fv3_fast_physics_fn)(void *user, const fv3_l2e_state *state);
+typedef struct {
+  int km, nq;
+  const double *ak, *bk;
+  double ptop, akap, cp, rdgas, grav, consv, t_min, p_ref, mdt;
+  int hydrostatic, fill, kord_mt, kord_wz, kord_tm, last_step;
+  const int *kord_tr; /* host array (nq) */
+  double *delp;
+  void *u, *v, *w, *delz, *pt, *q;
+  const void *ws;
+  double *pe, *peln, *pk, *pkz, *ps, *te0_2d, *dtmp_out;
+  fv3_fast_physics_fn fast_physics;
+  void *fast_physics_user;
+} fv3_l2e_args;
+int fv3_lagrangian_to_eulerian_dev(fv3_ctx *ctx, int form, const fv3_l2e_args *args, void *stream);
+/* GFDL_interstitial's kmp (ccpp/data/CCPP_typedefs.F90:1706-1718): the first
+ * level whose full-level pressure at p_ref exceeds 10 hPa (host arrays). */
+int fv3_kmp(int npz, double p_ref, const double *ak, const double *bk, int *kmp);
+
 /* Tangent-linear and adjoint of the FP64 advection + tracer remap step: the
  * derivatives of tracer_2d (per_level = 0; q_split > 0 forces nsplit) or
  * tracer_2d_1L (per_level = 1) followed by the tracer part of

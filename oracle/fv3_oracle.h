@@ -144,6 +144,34 @@ int fv3o_lagrangian_to_eulerian_tracers(int km, int nq, const fv3o_bounds *bd,
                                         const int *kord_tr, int kord_tm, double t_min,
                                         int fill);
 
+/* The rest of Lagrangian_to_Eulerian (fv3_oracle_l2e.c).  Arrays per tile,
+ * tiles outermost: delp, w, pt (isd:ied, jsd:jed, km); q (..., km, nq);
+ * u (isd:ied, jsd:jed+1, km) and v (isd:ied+1, jsd:jed, km) on the D grid;
+ * delz, pkz (is:ie, js:je, km); ws, te0_2d (is:ie, js:je); ps, area
+ * (isd:ied, jsd:jed); pe (is-1:ie+1, km+1, js-1:je+1); peln (is:ie, km+1,
+ * js:je); pk (is:ie, js:je, km+1).  delp must hold the Lagrangian thickness
+ * on the compute domain and its one-cell ring (the winds' interfaces average
+ * neighbouring columns).  w and delz are remapped when hydrostatic is 0;
+ * te0_2d (may be NULL) receives the column energy before the remap, and with
+ * consv > 0 the domain's energy loss is added back to pt (dtmp_out, may be
+ * NULL, receives the increment). */
+typedef struct {
+  int km, nq, ntiles;
+  fv3o_bounds bd;
+  const double *ak, *bk;
+  double ptop, akap, cp, rdgas, grav, consv, t_min;
+  int hydrostatic, fill, kord_mt, kord_wz, kord_tm;
+  const int *kord_tr;
+  double *delp, *u, *v, *w, *delz, *pt, *q;
+  const double *ws, *area;
+  double *pe, *peln, *pk, *pkz, *ps, *te0_2d, *dtmp_out;
+} fv3o_l2e;
+int fv3o_lagrangian_to_eulerian_full(const fv3o_l2e *a);
+
+/* OpenMP threads of the level (tracer_2d_1L) and row (remap) loops; n <= 0
+ * keeps the current count.  Returns the count in effect (1 without OpenMP). */
+int fv3o_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

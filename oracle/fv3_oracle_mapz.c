@@ -636,56 +636,61 @@ int fv3o_lagrangian_to_eulerian_tracers(int km, int nq, const fv3o_bounds *bd,
   const int is = bd->is, ie = bd->ie, im = ie - is + 1;
   const int isd = bd->isd, ied = bd->ied, jsd = bd->jsd, jed = bd->jed;
   const size_t ni = (size_t)(ied - isd + 1), plane = ni * (jed - jsd + 1);
-  double *pe1 = (double *)malloc(sizeof(double) * (size_t)im * (km + 1));
-  double *pe2 = (double *)malloc(sizeof(double) * (size_t)im * (km + 1));
-  double *dp2 = (double *)malloc(sizeof(double) * (size_t)im * km);
-  double *q2 = (double *)malloc(sizeof(double) * (size_t)im * km);
   int rc = 0;
+  /* rows are independent (upstream's OpenMP loop over j) */
+#pragma omp parallel reduction(| : rc)
+  {
+    double *pe1 = (double *)malloc(sizeof(double) * (size_t)im * (km + 1));
+    double *pe2 = (double *)malloc(sizeof(double) * (size_t)im * (km + 1));
+    double *dp2 = (double *)malloc(sizeof(double) * (size_t)im * km);
+    double *q2 = (double *)malloc(sizeof(double) * (size_t)im * km);
 #define PE1(i, k) pe1[(size_t)((k) - 1) * im + ((i) - is)]
 #define PE2(i, k) pe2[(size_t)((k) - 1) * im + ((i) - is)]
-  for (int j = bd->js; j <= bd->je && !rc; j++) {
-    for (int i = is; i <= ie; i++) PE1(i, 1) = ptop;
-    for (int k = 1; k <= km; k++)
-      for (int i = is; i <= ie; i++)
-        PE1(i, k + 1) = PE1(i, k) + delp[(size_t)(k - 1) * plane + (size_t)(j - jsd) * ni + (i - isd)];
-    for (int i = is; i <= ie; i++) {
-      PE2(i, 1) = ptop;
-      PE2(i, km + 1) = PE1(i, km + 1);
-    }
-    for (int k = 2; k <= km; k++)
-      for (int i = is; i <= ie; i++) PE2(i, k) = ak[k - 1] + bk[k - 1] * PE1(i, km + 1);
-    for (int k = 1; k <= km; k++)
-      for (int i = is; i <= ie; i++)
-        dp2[(size_t)(k - 1) * im + (i - is)] = PE2(i, k + 1) - PE2(i, k);
-
-    if (nq > 5) {
-      rc |= fv3o_mapn_tracer(nq, km, pe1, pe2, q, dp2, kord_tr, j, is, ie, isd, ied, jsd, jed,
-                             0.0, fill);
-    } else {
-      for (int iq = 1; iq <= nq && !rc; iq++) {
-        double *qq = q + (size_t)(iq - 1) * km * plane;
-        rc |= fv3o_map1_q2(km, pe1, qq, km, pe2, q2, dp2, is, ie, 0, kord_tr[iq - 1], j, isd,
-                           ied, jsd, jed, 0.0);
-        if (fill) fv3o_fillz(im, km, 1, q2, dp2);
-        for (int k = 1; k <= km; k++)
-          for (int i = is; i <= ie; i++)
-            qq[(size_t)(k - 1) * plane + (size_t)(j - jsd) * ni + (i - isd)] =
-                q2[(size_t)(k - 1) * im + (i - is)];
+#pragma omp for schedule(dynamic, 1)
+    for (int j = bd->js; j <= bd->je; j++) {
+      for (int i = is; i <= ie; i++) PE1(i, 1) = ptop;
+      for (int k = 1; k <= km; k++)
+        for (int i = is; i <= ie; i++)
+          PE1(i, k + 1) = PE1(i, k) + delp[(size_t)(k - 1) * plane + (size_t)(j - jsd) * ni + (i - isd)];
+      for (int i = is; i <= ie; i++) {
+        PE2(i, 1) = ptop;
+        PE2(i, km + 1) = PE1(i, km + 1);
       }
+      for (int k = 2; k <= km; k++)
+        for (int i = is; i <= ie; i++) PE2(i, k) = ak[k - 1] + bk[k - 1] * PE1(i, km + 1);
+      for (int k = 1; k <= km; k++)
+        for (int i = is; i <= ie; i++)
+          dp2[(size_t)(k - 1) * im + (i - is)] = PE2(i, k + 1) - PE2(i, k);
+
+      if (nq > 5) {
+        rc |= fv3o_mapn_tracer(nq, km, pe1, pe2, q, dp2, kord_tr, j, is, ie, isd, ied, jsd, jed,
+                               0.0, fill);
+      } else {
+        for (int iq = 1; iq <= nq; iq++) {
+          double *qq = q + (size_t)(iq - 1) * km * plane;
+          rc |= fv3o_map1_q2(km, pe1, qq, km, pe2, q2, dp2, is, ie, 0, kord_tr[iq - 1], j, isd,
+                             ied, jsd, jed, 0.0);
+          if (fill) fv3o_fillz(im, km, 1, q2, dp2);
+          for (int k = 1; k <= km; k++)
+            for (int i = is; i <= ie; i++)
+              qq[(size_t)(k - 1) * plane + (size_t)(j - jsd) * ni + (i - isd)] =
+                  q2[(size_t)(k - 1) * im + (i - is)];
+        }
+      }
+      if (pt != NULL)
+        rc |= fv3o_map_scalar(km, pe1, pt, NULL, km, pe2, pt, is, ie, j, isd, ied, jsd, jed, 1,
+                              kord_tm, t_min);
+      for (int k = 1; k <= km; k++)
+        for (int i = is; i <= ie; i++)
+          delp[(size_t)(k - 1) * plane + (size_t)(j - jsd) * ni + (i - isd)] =
+              dp2[(size_t)(k - 1) * im + (i - is)];
     }
-    if (pt != NULL && !rc)
-      rc |= fv3o_map_scalar(km, pe1, pt, NULL, km, pe2, pt, is, ie, j, isd, ied, jsd, jed, 1,
-                            kord_tm, t_min);
-    for (int k = 1; k <= km; k++)
-      for (int i = is; i <= ie; i++)
-        delp[(size_t)(k - 1) * plane + (size_t)(j - jsd) * ni + (i - isd)] =
-            dp2[(size_t)(k - 1) * im + (i - is)];
-  }
 #undef PE1
 #undef PE2
-  free(pe1);
-  free(pe2);
-  free(dp2);
-  free(q2);
+    free(pe1);
+    free(pe2);
+    free(dp2);
+    free(q2);
+  }
   return rc;
 }

@@ -571,6 +571,7 @@ static void tr_update(double *q_out, const double *q_in, const double *dp1k, con
 typedef struct {
   tr_dims d;
   size_t tq, tdp, tcx, tcy, tmx, tmy; /* per-tile strides */
+  int npz;
 } tr_tiles;
 
 static tr_tiles tr_tiles_of(const fv3o_bounds *bd, int npz, int nq) {
@@ -582,7 +583,80 @@ static tr_tiles tr_tiles_of(const fv3o_bounds *bd, int npz, int nq) {
   t.tcy = t.d.sy * npz;
   t.tmx = t.d.smx * npz;
   t.tmy = t.d.smy * npz;
+  t.npz = npz;
   return t;
+}
+
+/* One level of tracer_2d_1L: nsplt sub-cycles of every tile and tracer.
+ * Work buffers: ra_x, ra_y, dp2 (per tile), qn2 (per tile and tracer), fx, fy. */
+typedef struct {
+  double *qn2, *dp2, *ra_x, *ra_y, *fx, *fy;
+} tr_work;
+
+static int tr_alloc(tr_work *w, const fv3o_bounds *bd, int ntiles, int nq) {
+  const tr_dims d = tr_dims_of(bd);
+  const size_t nj = (size_t)(bd->je - bd->js + 1);
+  w->qn2 = (double *)malloc(sizeof(double) * d.s2 * nq * ntiles);
+  w->dp2 = (double *)malloc(sizeof(double) * d.niy * nj * ntiles);
+  w->ra_x = (double *)malloc(sizeof(double) * d.niy * d.njd * ntiles);
+  w->ra_y = (double *)malloc(sizeof(double) * d.nid * nj * ntiles);
+  w->fx = (double *)malloc(sizeof(double) * d.smx);
+  w->fy = (double *)malloc(sizeof(double) * d.smy);
+  return w->qn2 && w->dp2 && w->ra_x && w->ra_y && w->fx && w->fy ? 0 : 1;
+}
+
+static void tr_free(tr_work *w) {
+  free(w->qn2);
+  free(w->dp2);
+  free(w->ra_x);
+  free(w->ra_y);
+  free(w->fx);
+  free(w->fy);
+}
+
+static int tr_level_1L(int k, int nsplt, double *q, double *dp1, double *mfx, double *mfy,
+                       double *cx, double *cy, const double *xfx, const double *yfx,
+                       const fv3o_grid *g, const fv3o_bounds *bd, const tr_tiles *T, int ntiles,
+                       int npx, int npy, int nq, int hord, double lim_fac, fv3o_halo_fn halo,
+                       void *halo_ctx, tr_work *w) {
+  const tr_dims d = T->d;
+  const size_t sdp2 = (size_t)d.niy * (bd->je - bd->js + 1);
+  const size_t srax = (size_t)d.niy * d.njd, sray = (size_t)d.nid * (bd->je - bd->js + 1);
+  int rc = 0;
+  for (int t = 0; t < ntiles; t++)
+    tr_ra(xfx + t * T->tcx + (k - 1) * d.sx, yfx + t * T->tcy + (k - 1) * d.sy, w->ra_x + t * srax,
+          w->ra_y + t * sray, &g[t], bd);
+  for (int it = 1; it <= nsplt && !rc; it++) {
+    for (int t = 0; t < ntiles; t++)
+      tr_dp2(dp1 + t * T->tdp + (k - 1) * d.s2, mfx + t * T->tmx + (k - 1) * d.smx,
+             mfy + t * T->tmy + (k - 1) * d.smy, w->dp2 + t * sdp2, &g[t], bd);
+    for (int t = 0; t < ntiles && !rc; t++) {
+      double *dp1k = dp1 + t * T->tdp + (k - 1) * d.s2;
+      double *cxk = cx + t * T->tcx + (k - 1) * d.sx, *cyk = cy + t * T->tcy + (k - 1) * d.sy;
+      const double *xk = xfx + t * T->tcx + (k - 1) * d.sx, *yk = yfx + t * T->tcy + (k - 1) * d.sy;
+      double *mxk = mfx + t * T->tmx + (k - 1) * d.smx, *myk = mfy + t * T->tmy + (k - 1) * d.smy;
+      for (int iq = 1; iq <= nq && !rc; iq++) {
+        double *qk = q + t * T->tq + ((size_t)(iq - 1) * T->npz + (k - 1)) * d.s2;
+        double *qn = w->qn2 + ((size_t)t * nq + (iq - 1)) * d.s2;
+        double *work = nsplt != 1 ? qn : qk;
+        if (nsplt != 1 && it == 1) memcpy(qn, qk, sizeof(double) * d.s2);
+        rc |= fv3o_fv_tp_2d(work, cxk, cyk, npx, npy, hord, w->fx, w->fy, xk, yk, &g[t], bd,
+                            w->ra_x + t * srax, w->ra_y + t * sray, lim_fac, mxk, myk, NULL, NULL,
+                            NULL);
+        tr_update(it < nsplt ? qn : qk, work, dp1k, w->fx, w->fy, w->dp2 + t * sdp2, &g[t], bd);
+      }
+    }
+    if (it < nsplt) {
+      for (int t = 0; t < ntiles; t++) {
+        double *dp1k = dp1 + t * T->tdp + (k - 1) * d.s2;
+        for (int j = bd->js; j <= bd->je; j++)
+          for (int i = bd->is; i <= bd->ie; i++)
+            E2(dp1k, i, j, bd->isd, bd->jsd, d.nid) = E2(w->dp2 + t * sdp2, i, j, bd->is, bd->js, d.niy);
+      }
+      if (halo) halo(halo_ctx, w->qn2, nq, k); /* mpp_update_domains(qn2) */
+    }
+  }
+  return rc;
 }
 
 int fv3o_tracer_2d_1L(double *q, double *dp1, double *mfx, double *mfy, double *cx,
@@ -602,14 +676,6 @@ int fv3o_tracer_2d_1L(double *q, double *dp1, double *mfx, double *mfy, double *
   double *yfx = (double *)malloc(sizeof(double) * T.tcy * ntiles);
   double *cmax = (double *)calloc((size_t)npz * ntiles, sizeof(double));
   double *cm = (double *)calloc(npz, sizeof(double));
-  double *qn2 = (double *)malloc(sizeof(double) * d.s2 * nq * ntiles);
-  double *dp2 = (double *)malloc(sizeof(double) * d.niy * (bd->je - bd->js + 1) * ntiles);
-  double *ra_x = (double *)malloc(sizeof(double) * d.niy * d.njd * ntiles);
-  double *ra_y = (double *)malloc(sizeof(double) * d.nid * (bd->je - bd->js + 1) * ntiles);
-  double *fx = (double *)malloc(sizeof(double) * d.smx);
-  double *fy = (double *)malloc(sizeof(double) * d.smy);
-  const size_t sdp2 = (size_t)d.niy * (bd->je - bd->js + 1);
-  const size_t srax = (size_t)d.niy * d.njd, sray = (size_t)d.nid * (bd->je - bd->js + 1);
 
   for (int t = 0; t < ntiles; t++)
     tr_area_fluxes(cx + t * T.tcx, cy + t * T.tcy, xfx + t * T.tcx, yfx + t * T.tcy,
@@ -630,52 +696,25 @@ int fv3o_tracer_2d_1L(double *q, double *dp1, double *mfx, double *mfy, double *
   }
   if (halo) halo(halo_ctx, q, npz * nq, 0); /* complete_group_halo_update(q_pack) */
 
-  for (int k = 1; k <= npz && !rc; k++) {
-    for (int t = 0; t < ntiles; t++)
-      tr_ra(xfx + t * T.tcx + (k - 1) * d.sx, yfx + t * T.tcy + (k - 1) * d.sy,
-            ra_x + t * srax, ra_y + t * sray, &g[t], bd);
-    int nsplt = (int)(1.0 + cm[k - 1]);
-    for (int it = 1; it <= nsplt && !rc; it++) {
-      for (int t = 0; t < ntiles; t++)
-        tr_dp2(dp1 + t * T.tdp + (k - 1) * d.s2, mfx + t * T.tmx + (k - 1) * d.smx,
-               mfy + t * T.tmy + (k - 1) * d.smy, dp2 + t * sdp2, &g[t], bd);
-      for (int t = 0; t < ntiles && !rc; t++) {
-        double *dp1k = dp1 + t * T.tdp + (k - 1) * d.s2;
-        double *cxk = cx + t * T.tcx + (k - 1) * d.sx, *cyk = cy + t * T.tcy + (k - 1) * d.sy;
-        double *xk = xfx + t * T.tcx + (k - 1) * d.sx, *yk = yfx + t * T.tcy + (k - 1) * d.sy;
-        double *mxk = mfx + t * T.tmx + (k - 1) * d.smx, *myk = mfy + t * T.tmy + (k - 1) * d.smy;
-        for (int iq = 1; iq <= nq && !rc; iq++) {
-          double *qk = q + t * T.tq + ((size_t)(iq - 1) * npz + (k - 1)) * d.s2;
-          double *qn = qn2 + ((size_t)t * nq + (iq - 1)) * d.s2;
-          double *work = nsplt != 1 ? qn : qk;
-          if (nsplt != 1 && it == 1) memcpy(qn, qk, sizeof(double) * d.s2);
-          rc |= fv3o_fv_tp_2d(work, cxk, cyk, npx, npy, hord, fx, fy, xk, yk, &g[t], bd,
-                              ra_x + t * srax, ra_y + t * sray, lim_fac, mxk, myk, NULL, NULL,
-                              NULL);
-          tr_update(it < nsplt ? qn : qk, work, dp1k, fx, fy, dp2 + t * sdp2, &g[t], bd);
-        }
-      }
-      if (it < nsplt) {
-        for (int t = 0; t < ntiles; t++) {
-          double *dp1k = dp1 + t * T.tdp + (k - 1) * d.s2;
-          for (int j = bd->js; j <= bd->je; j++)
-            for (int i = bd->is; i <= bd->ie; i++)
-              E2(dp1k, i, j, bd->isd, bd->jsd, d.nid) = E2(dp2 + t * sdp2, i, j, bd->is, bd->js, d.niy);
-        }
-        if (halo) halo(halo_ctx, qn2, nq, k); /* mpp_update_domains(qn2) */
-      }
+  /* Levels are independent (upstream's OpenMP loop over k).  A halo callback
+   * (six tiles) runs on the calling thread, so that case stays serial. */
+#pragma omp parallel if (halo == NULL) reduction(| : rc)
+  {
+    tr_work w;
+    if (tr_alloc(&w, bd, ntiles, nq)) {
+      rc |= 1;
+    } else {
+#pragma omp for schedule(dynamic, 1)
+      for (int k = 1; k <= npz; k++)
+        rc |= tr_level_1L(k, (int)(1.0 + cm[k - 1]), q, dp1, mfx, mfy, cx, cy, xfx, yfx, g, bd, &T,
+                          ntiles, npx, npy, nq, hord, lim_fac, halo, halo_ctx, &w);
     }
+    tr_free(&w);
   }
   free(xfx);
   free(yfx);
   free(cmax);
   free(cm);
-  free(qn2);
-  free(dp2);
-  free(ra_x);
-  free(ra_y);
-  free(fx);
-  free(fy);
   return rc;
 }
 
@@ -769,4 +808,19 @@ int fv3o_tracer_2d(double *q, double *dp1, double *mfx, double *mfy, double *cx,
   free(fx);
   free(fy);
   return rc;
+}
+
+/* OpenMP threads of the oracle's level and row loops (the CPU baseline times
+ * one thread and all of them). */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int fv3o_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
 }

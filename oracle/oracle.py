@@ -35,6 +35,14 @@ class GridC(ctypes.Structure):
 HALO_FN = ctypes.CFUNCTYPE(None, P, P, I, I)
 
 
+class L2E(ctypes.Structure):
+    _fields_ = [("km", I), ("nq", I), ("ntiles", I), ("bd", Bounds), ("ak", P), ("bk", P)] + \
+               [(k, D) for k in ("ptop", "akap", "cp", "rdgas", "grav", "consv", "t_min")] + \
+               [(k, I) for k in ("hydrostatic", "fill", "kord_mt", "kord_wz", "kord_tm")] + \
+               [(k, P) for k in ("kord_tr", "delp", "u", "v", "w", "delz", "pt", "q", "ws", "area", "pe",
+                                 "peln", "pk", "pkz", "ps", "te0_2d", "dtmp_out")]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -58,6 +66,9 @@ def lib():
         L.fv3o_mapn_tracer.argtypes = [I, I, P, P, P, P, P, I, I, I, I, I, I, I, D, I]
         L.fv3o_fillz.argtypes = [I, I, I, P, P]
         L.fv3o_lagrangian_to_eulerian_tracers.argtypes = [I, I, P, P, P, D, P, P, P, P, I, D, I]
+        L.fv3o_lagrangian_to_eulerian_full.argtypes = [ctypes.POINTER(L2E)]
+        L.fv3o_set_threads.argtypes = [I]
+        L.fv3o_set_threads.restype = I
         _lib = L
     return _lib
 
@@ -275,3 +286,35 @@ def copy_corners(q, n, dir):
     bd = bounds(n)
     lib().fv3o_copy_corners(_p(q), n + 1, n + 1, dir, ctypes.byref(bd), ctypes.byref(g))
     return q
+
+
+def lagrangian_to_eulerian_full(n, ntiles, km, nq, ak, bk, ptop, st, kord_tr, kord_tm, kord_mt, kord_wz,
+                                hydrostatic, consv, areas, akap=287.05 / 1004.6, cp=1004.6, rdgas=287.05,
+                                grav=9.80665, t_min=184.0, fill=False, with_te=True):
+    """The full Lagrangian_to_Eulerian on copies of the state dict ``st``
+    (delp, u, v, w, delz, pt, q, ws, pkz: Fortran-order float64 arrays with
+    the tile as the last axis when ntiles > 1).  Returns (new state dict with
+    pe, peln, pk, ps, te0_2d added, dtmp)."""
+    out = {k: np.array(v, order="F", copy=True) for k, v in st.items()}
+    nd, ni = n + 2 * NG, n
+    shp = lambda *s: s + ((ntiles,) if ntiles > 1 else ())
+    out["pe"] = np.zeros(shp(ni + 2, km + 1, ni + 2), order="F")
+    out["peln"] = np.zeros(shp(ni, km + 1, ni), order="F")
+    out["pk"] = np.zeros(shp(ni, ni, km + 1), order="F")
+    out["ps"] = np.zeros(shp(nd, nd), order="F")
+    out["te0_2d"] = np.zeros(shp(ni, ni), order="F")
+    area = np.asfortranarray(np.stack(areas, axis=-1) if ntiles > 1 else areas[0])
+    kt = np.full(max(nq, 1), kord_tr, dtype=np.int32)
+    dt = ctypes.c_double(0.0)
+    a = L2E(km=km, nq=nq, ntiles=ntiles, bd=bounds(n), ak=_p(np.ascontiguousarray(ak)),
+            bk=_p(np.ascontiguousarray(bk)), ptop=ptop, akap=akap, cp=cp, rdgas=rdgas, grav=grav,
+            consv=consv, t_min=t_min, hydrostatic=int(hydrostatic), fill=int(fill), kord_mt=kord_mt,
+            kord_wz=kord_wz, kord_tm=kord_tm, kord_tr=_p(kt), delp=_p(out["delp"]), u=_p(out["u"]),
+            v=_p(out["v"]), w=_p(out.get("w")), delz=_p(out.get("delz")), pt=_p(out["pt"]),
+            q=_p(out.get("q")), ws=_p(out.get("ws")), area=_p(area), pe=_p(out["pe"]),
+            peln=_p(out["peln"]), pk=_p(out["pk"]), pkz=_p(out["pkz"]), ps=_p(out["ps"]),
+            te0_2d=_p(out["te0_2d"]) if with_te else None,
+            dtmp_out=ctypes.cast(ctypes.pointer(dt), P))
+    if lib().fv3o_lagrangian_to_eulerian_full(ctypes.byref(a)):
+        raise ValueError("oracle Lagrangian_to_Eulerian failed")
+    return out, dt.value
