@@ -60,6 +60,9 @@ SIGNATURES = {
     "fv3_fv_tp_2d": (I, [P, P, P, P, I, I, I, P, P, P, P, P, P, D, P, P, P, IP, DP]),
     "fv3_tracer_2d_1l": (I, [P, P, P, P, P, P, P, I, I, I, I, I, I, D, I, I, D, D, P]),
     "fv3_tracer_2d": (I, [P, P, P, P, P, P, P, I, I, I, I, I, I, D, I, I, D, D, P]),
+    "fv3_tracer_2d_1l_r4": (I, [P, P, P, P, P, P, P, I, I, I, I, I, I, D, I, I, D, D, P]),
+    "fv3_tracer_2d_r4": (I, [P, P, P, P, P, P, P, I, I, I, I, I, I, D, I, I, D, D, P]),
+    "fv3_lagrangian_to_eulerian_tracers_r4": (I, [P, I, I, P, P, D, P, P, P, P, I, D, I]),
     "fv3_map1_ppm": (I, [I, P, P, P, I, P, P, I, I, I, I, I, I, I, I, I]),
     "fv3_map_scalar": (I, [I, P, P, P, I, P, P, I, I, I, I, I, I, I, I, I, D]),
     "fv3_mapn_tracer": (I, [I, I, P, P, P, P, P, I, I, I, I, I, I, I, D, I]),

@@ -89,26 +89,31 @@ def build_cases():
 
 def build_fortran():
     """The ISO_C_BINDING layer (Fortran interfaces of the path) and its
-    harness executable; skipped when amdflang is absent."""
+    harness executable, for the 8-byte build and for the 32-bit build
+    (OVERLOAD_R4: 4-byte reals, the FP32 kernels); skipped when amdflang is
+    absent."""
     fdir = os.path.join(HERE, "fortran")
     srcs = ["fv_arrays_stub.F90", "fv3atm_amd_cabi.F90", "tp_core_mod.F90", "fv_tracer2d_mod.F90",
             "fv_mapz_mod.F90"]
     if not os.path.exists(FC) or not all(os.path.exists(os.path.join(fdir, s)) for s in srcs):
         return None
-    mod = os.path.join(OBJ, "fmod")
-    os.makedirs(mod, exist_ok=True)
-    lib = os.path.join(HERE, "libfv3atm_amd_f.so")
-    _run([FC, "-O2", "-fPIC", "-shared", "-module-dir", mod, "-o", lib + ".tmp",
-          *[os.path.join(fdir, s) for s in srcs], "-L" + HERE, "-lfv3atm_amd",
-          "-Wl,-rpath,$ORIGIN"])
-    os.replace(lib + ".tmp", lib)
-    harness = os.path.join(fdir, "harness.F90")
-    if os.path.exists(harness):
-        exe = os.path.join(HERE, "fv3_harness")
-        _run([FC, "-O2", "-I" + mod, "-o", exe + ".tmp", harness, "-L" + HERE, "-lfv3atm_amd_f",
-              "-lfv3atm_amd", "-Wl,-rpath,$ORIGIN"])
-        os.replace(exe + ".tmp", exe)
-    return lib
+    out = []
+    for tag, defs in (("", []), ("_r4", ["-DOVERLOAD_R4"])):
+        mod = os.path.join(OBJ, "fmod" + tag)
+        os.makedirs(mod, exist_ok=True)
+        lib = os.path.join(HERE, f"libfv3atm_amd_f{tag}.so")
+        _run([FC, "-O2", "-fPIC", "-shared", *defs, "-module-dir", mod, "-o", lib + ".tmp",
+              *[os.path.join(fdir, s) for s in srcs], "-L" + HERE, "-lfv3atm_amd",
+              "-Wl,-rpath,$ORIGIN"])
+        os.replace(lib + ".tmp", lib)
+        harness = os.path.join(fdir, "harness.F90")
+        if os.path.exists(harness):
+            exe = os.path.join(HERE, "fv3_harness" + tag)
+            _run([FC, "-O2", *defs, "-I" + mod, "-o", exe + ".tmp", harness, "-L" + HERE,
+                  f"-lfv3atm_amd_f{tag}", "-lfv3atm_amd", "-Wl,-rpath,$ORIGIN"])
+            os.replace(exe + ".tmp", exe)
+        out.append(lib)
+    return " ".join(out)
 
 
 def build_oracle():

@@ -436,6 +436,162 @@ int fv3_fv_tp_2d(fv3_ctx *h, double *q, const double *crx, const double *cry, in
   return stage_back(c, arrs, dv);
 }
 
+}  // extern "C"
+
+// ---- the host-array transport through the fused kernels ------------------------
+// After a fused transport (which reads the winds and dp1 without changing
+// them), the in-place semantics of tracer_2d: the Courant numbers and mass
+// fluxes scaled by 1/nsplit(k), dp1 advanced to the thickness at the start
+// of the last sub-cycle (the oracle's running sum, in its order).
+template <typename T>
+__global__ void k_scale_levels(T *a, long long per_level, int npz, int nt, const int *nsplit) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= per_level * npz * nt) return;
+  const int k = (int)((idx / per_level) % npz);
+  const int ns = nsplit[k];
+  if (ns > 1) {
+    const double frac = 1.0 / (double)ns;
+    a[idx] = a[idx] * (T)frac;
+  }
+}
+
+template <typename T>
+__global__ void k_dp1_advance(T *dp1, const T *mfx, const T *mfy, const double *rarea, Dims d, int npz, int nt,
+                              const int *nsplit, long long grs2) {
+  const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)ni * nj * npz * nt) return;
+  const int i = d.is + (int)(idx % ni), j = d.js + (int)((idx / ni) % nj);
+  const int k = (int)((idx / ((long long)ni * nj)) % npz), t = (int)(idx / ((long long)ni * nj * npz));
+  const int ns = nsplit[k];
+  if (ns < 2) return;
+  const long long z = (long long)t * npz + k;
+  const T *mx = mfx + z * d.smx + (long long)(j - d.js) * d.nix;
+  const T *my = mfy + z * d.smy;
+  const double ra = rarea[t * grs2 + (long long)(j - d.jsd) * d.nid + (i - d.isd)];
+  T *p = dp1 + z * d.s2 + (long long)(j - d.jsd) * d.nid + (i - d.isd);
+  T v = *p;
+  for (int it = 1; it < ns; it++)
+    v = v + (T)(((mx[i - d.is] - mx[i + 1 - d.is]) +
+                 (my[(long long)(j - d.js) * d.niy + (i - d.is)] - my[(long long)(j + 1 - d.js) * d.niy + (i - d.is)])) *
+                (T)ra);
+  *p = v;
+}
+
+// fv_tp_2d runs copy_corners on its q argument (tp_core.F90 fv_tp_2d: dir 2
+// before the y sweep, dir 1 before the x sweep), so a level the caller's q
+// went through in place leaves the transport with its cube-corner halo
+// blocks holding the dir-1 substitution of the adjacent halo strips: every
+// level of tracer_2d, and the levels with nsplit = 1 of tracer_2d_1L (whose
+// sub-cycled levels run on a copy, fv_tracer2d.F90 tracer_2d_1L qn2).  The
+// strips themselves are not written by the transport.
+template <typename T>
+__global__ void k_corner_blocks(T *q, Dims d, int npz, int nq, int nt, const int *nsplit, int per_level) {
+  constexpr int nb = NG * NG;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 4LL * nb * npz * nq * nt) return;
+  const int cell = (int)(idx % (4 * nb)), corner = cell / nb, a = cell % NG, b = (cell / NG) % NG;
+  const long long lev = idx / (4 * nb);
+  const int k = (int)(lev % npz);
+  if (per_level && nsplit[k] != 1) return;
+  const int i = (corner == 1 || corner == 2) ? d.npx + a : 1 - NG + a;
+  const int j = corner >= 2 ? d.npy + b : 1 - NG + b;
+  int si, sj;
+  if (!fv3::corner_src(i, j, d.npx, d.npy, 1, d.corners, si, sj)) return;
+  if (i < d.isd || i > d.ied || j < d.jsd || j > d.jed || si < d.isd || si > d.ied || sj < d.jsd || sj > d.jed)
+    return;
+  T *p = q + lev * d.s2;
+  p[(long long)(j - d.jsd) * d.nid + (i - d.isd)] = p[(long long)(sj - d.jsd) * d.nid + (si - d.isd)];
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A *a, B *b, long long n) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) b[idx] = (B)a[idx];
+}
+
+static dim3 grid_of_n(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// bytes the fused step keeps at the front of the staging arena (Courant
+// maxima, fractions, sub-cycle counts, flag)
+static size_t step_head_bytes(int npz) { return 4 * (((size_t)8 * npz + 255) & ~(size_t)255) + 4096; }
+
+// tracer_2d / tracer_2d_1L on host arrays of T through the fused transport
+template <typename T>
+static int fused_transport_host(Ctx &c, bool per_level, int q_split, T *q, T *dp1, T *mfx, T *mfy, T *cx,
+                                T *cy, int nq, int hord, int *nsplit_out) {
+  const Dims &d = c.d;
+  const int npz = c.npz, nt = c.ntiles;
+  hipStream_t s = c.stream;
+  const long long nqv = d.s2 * npz * nq * nt, ndp = d.s2 * npz * nt;
+  const long long ncx = d.sx * npz * nt, ncy = d.sy * npz * nt, nmx = d.smx * npz * nt, nmy = d.smy * npz * nt;
+  size_t off = step_head_bytes(npz);
+  auto take = [&](size_t b) {
+    size_t o = off;
+    off += (b + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t oq = take(sizeof(T) * nqv), oqo = take(sizeof(T) * nqv), odp = take(sizeof(T) * ndp);
+  const size_t odd = take(sizeof(double) * ndp), ocx = take(sizeof(T) * ncx), ocy = take(sizeof(T) * ncy);
+  const size_t omx = take(sizeof(T) * nmx), omy = take(sizeof(T) * nmy), ons = take(sizeof(int) * npz);
+  char *base = (char *)ctx_stage(c, off);
+  if (!base) return 1;
+  T *dq = (T *)(base + oq), *dqo = (T *)(base + oqo), *ddp = (T *)(base + odp);
+  double *ddd = (double *)(base + odd);
+  T *dcx = (T *)(base + ocx), *dcy = (T *)(base + ocy), *dmx = (T *)(base + omx), *dmy = (T *)(base + omy);
+  int *dns = (int *)(base + ons);
+  FV3_HIP(hipMemcpyAsync(dq, q, sizeof(T) * nqv, hipMemcpyHostToDevice, s));
+  FV3_HIP(hipMemcpyAsync(ddp, dp1, sizeof(T) * ndp, hipMemcpyHostToDevice, s));
+  FV3_HIP(hipMemcpyAsync(dcx, cx, sizeof(T) * ncx, hipMemcpyHostToDevice, s));
+  FV3_HIP(hipMemcpyAsync(dcy, cy, sizeof(T) * ncy, hipMemcpyHostToDevice, s));
+  FV3_HIP(hipMemcpyAsync(dmx, mfx, sizeof(T) * nmx, hipMemcpyHostToDevice, s));
+  FV3_HIP(hipMemcpyAsync(dmy, mfy, sizeof(T) * nmy, hipMemcpyHostToDevice, s));
+  // the step writes the interior of q_out only: its halo keeps q's
+  FV3_HIP(hipMemcpyAsync(dqo, dq, sizeof(T) * nqv, hipMemcpyDeviceToDevice, s));
+  const double *dp1d = (const double *)ddp;
+  if (sizeof(T) != sizeof(double)) {
+    hipLaunchKernelGGL((k_convert<T, double>), grid_of_n(ndp), dim3(256), 0, s, ddp, ddd, ndp);
+    FV3_HIP(hipGetLastError());
+    dp1d = ddd;
+  }
+  const bool saved_pl = c.step_per_level;
+  const int saved_qs = c.step_q_split;
+  c.step_per_level = per_level;
+  c.step_q_split = per_level ? 0 : q_split;
+  c.transport_only = true;
+  const int rc = fused_step(c, sizeof(T) == 4, nq, dq, dqo, nullptr, dp1d, nullptr, dcx, dcy, dmx, dmy, nullptr,
+                            nullptr, 0.0, hord, 9, 10, 184.0, 0);
+  c.transport_only = false;
+  c.step_per_level = saved_pl;
+  c.step_q_split = saved_qs;
+  if (rc) return rc;
+  if ((int)c.last_nsplit.size() != npz) return set_error("sub-cycle counts of the transport missing");
+  FV3_HIP(hipMemcpyAsync(dns, c.last_nsplit.data(), sizeof(int) * npz, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_scale_levels<T>, grid_of_n(ncx), dim3(256), 0, s, dcx, d.sx, npz, nt, dns);
+  hipLaunchKernelGGL(k_scale_levels<T>, grid_of_n(ncy), dim3(256), 0, s, dcy, d.sy, npz, nt, dns);
+  hipLaunchKernelGGL(k_scale_levels<T>, grid_of_n(nmx), dim3(256), 0, s, dmx, d.smx, npz, nt, dns);
+  hipLaunchKernelGGL(k_scale_levels<T>, grid_of_n(nmy), dim3(256), 0, s, dmy, d.smy, npz, nt, dns);
+  const long long ncell = (long long)(d.ie - d.is + 1) * (d.je - d.js + 1) * npz * nt;
+  hipLaunchKernelGGL(k_dp1_advance<T>, grid_of_n(ncell), dim3(256), 0, s, ddp, dmx, dmy, c.grid64.rarea, d, npz,
+                     nt, dns, c.grid64.s2);
+  if (!d.bounded)
+    hipLaunchKernelGGL(k_corner_blocks<T>, grid_of_n(4LL * NG * NG * npz * nq * nt), dim3(256), 0, s, dqo, d, npz,
+                       nq, nt, dns, per_level ? 1 : 0);
+  FV3_HIP(hipGetLastError());
+  FV3_HIP(hipMemcpyAsync(q, dqo, sizeof(T) * nqv, hipMemcpyDeviceToHost, s));
+  FV3_HIP(hipMemcpyAsync(dp1, ddp, sizeof(T) * ndp, hipMemcpyDeviceToHost, s));
+  FV3_HIP(hipMemcpyAsync(cx, dcx, sizeof(T) * ncx, hipMemcpyDeviceToHost, s));
+  FV3_HIP(hipMemcpyAsync(cy, dcy, sizeof(T) * ncy, hipMemcpyDeviceToHost, s));
+  FV3_HIP(hipMemcpyAsync(mfx, dmx, sizeof(T) * nmx, hipMemcpyDeviceToHost, s));
+  FV3_HIP(hipMemcpyAsync(mfy, dmy, sizeof(T) * nmy, hipMemcpyDeviceToHost, s));
+  FV3_HIP(hipStreamSynchronize(s));
+  if (nsplit_out)
+    for (int k = 0; k < npz; k++) nsplit_out[k] = c.last_nsplit[k];
+  return 0;
+}
+
+extern "C" {
+
 static int tracer_common(fv3_ctx *h, bool per_level, double *q, double *dp1, double *mfx,
                          double *mfy, double *cx, double *cy, int npx, int npy, int npz, int nq,
                          int hord, int q_split, double dt, int id_divg, int nord_tr, double trdm,
@@ -446,6 +602,16 @@ static int tracer_common(fv3_ctx *h, bool per_level, double *q, double *dp1, dou
   if (!q || !dp1 || !mfx || !mfy || !cx || !cy) return set_error("tracer_2d: NULL array argument");
   if (nq < 1) return set_error("nq must be >= 1 (got %d)", nq);
   if (q_split < 0) return set_error("q_split must be >= 0 (got %d)", q_split);
+  // the fused FP64 transport (bit for bit the plain form's results) unless
+  // the call asks for what only the plain form does: deln_flux damping on
+  // the first sub-cycle, dp1 overwritten with the divergence (id_divg), or
+  // tracer_2d on several faces, whose q halo the reference leaves holding
+  // the exchange before the last sub-cycle
+  const bool damp = !per_level && trdm > 1.0e-4;
+  const bool divg = !per_level && id_divg > 0;
+  const bool faces = !per_level && c.ntiles > 1;
+  if (!damp && !divg && !faces && hord >= 7 && hord <= 13)
+    return fused_transport_host<double>(c, per_level, q_split, q, dp1, mfx, mfy, cx, cy, nq, hord, nsplit_out);
   const Dims &d = c.d;
   const long long nt = c.ntiles;
   std::vector<HostArr> arrs = {
@@ -475,6 +641,90 @@ int fv3_tracer_2d(fv3_ctx *h, double *q, double *dp1, double *mfx, double *mfy, 
                   int id_divg, int nord_tr, double trdm, double lim_fac, int *nsplit_out) {
   return tracer_common(h, false, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, hord, q_split, dt,
                        id_divg, nord_tr, trdm, lim_fac, nsplit_out);
+}
+
+static int tracer_common_r4(fv3_ctx *h, bool per_level, float *q, float *dp1, float *mfx, float *mfy,
+                            float *cx, float *cy, int npx, int npy, int npz, int nq, int hord, int q_split,
+                            int id_divg, int nord_tr, double trdm, int *nsplit_out);
+
+int fv3_tracer_2d_1l_r4(fv3_ctx *h, float *q, float *dp1, float *mfx, float *mfy, float *cx, float *cy,
+                        int npx, int npy, int npz, int nq, int hord, int q_split, double dt, int id_divg,
+                        int nord_tr, double trdm, double lim_fac, int *nsplit_out) {
+  (void)dt;
+  (void)lim_fac;
+  return tracer_common_r4(h, true, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, hord, q_split, id_divg,
+                          nord_tr, trdm, nsplit_out);
+}
+
+int fv3_tracer_2d_r4(fv3_ctx *h, float *q, float *dp1, float *mfx, float *mfy, float *cx, float *cy, int npx,
+                     int npy, int npz, int nq, int hord, int q_split, double dt, int id_divg, int nord_tr,
+                     double trdm, double lim_fac, int *nsplit_out) {
+  (void)dt;
+  (void)lim_fac;
+  return tracer_common_r4(h, false, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, hord, q_split, id_divg,
+                          nord_tr, trdm, nsplit_out);
+}
+
+int fv3_lagrangian_to_eulerian_tracers_r4(fv3_ctx *h, int km, int nq, const double *ak, const double *bk,
+                                          double ptop, float *delp, float *q, float *pt, const int *kord_tr,
+                                          int kord_tm, double t_min, int fill) {
+  if (!h) return set_error("NULL context");
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (km != c.npz) return set_error("km=%d differs from the context npz=%d", km, c.npz);
+  if (!ak || !bk || !delp || nq < 1 || !q || !kord_tr) return set_error("remap: NULL array argument or nq < 1");
+  if (fill) return set_error("the 32-bit remap has no fillz (fill must be 0)");
+  for (int iq = 1; iq < nq; iq++)
+    if (kord_tr[iq] != kord_tr[0]) return set_error("the 32-bit remap takes one kord_tr for every tracer");
+  const Dims &d = c.d;
+  const int nt = c.ntiles;
+  hipStream_t s = c.stream;
+  const long long nqv = d.s2 * km * nq * nt, ndp = d.s2 * km * nt;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    size_t o = off;
+    off += (b + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t oq = take(sizeof(float) * nqv), oqo = take(sizeof(float) * nqv), odp = take(sizeof(float) * ndp);
+  const size_t od1 = take(sizeof(double) * ndp), od2 = take(sizeof(double) * ndp);
+  const size_t opt = take(sizeof(float) * ndp), oab = take(sizeof(double) * 2 * (km + 1));
+  char *base = (char *)ctx_stage(c, off);
+  if (!base) return 1;
+  float *dq = (float *)(base + oq), *dqo = (float *)(base + oqo), *ddp = (float *)(base + odp);
+  double *dp1d = (double *)(base + od1), *dp2d = (double *)(base + od2), *dab = (double *)(base + oab);
+  float *dpt = pt ? (float *)(base + opt) : nullptr;
+  FV3_HIP(hipMemcpyAsync(dq, q, sizeof(float) * nqv, hipMemcpyHostToDevice, s));
+  FV3_HIP(hipMemcpyAsync(dqo, dq, sizeof(float) * nqv, hipMemcpyDeviceToDevice, s));
+  FV3_HIP(hipMemcpyAsync(ddp, delp, sizeof(float) * ndp, hipMemcpyHostToDevice, s));
+  if (pt) FV3_HIP(hipMemcpyAsync(dpt, pt, sizeof(float) * ndp, hipMemcpyHostToDevice, s));
+  FV3_HIP(hipMemcpyAsync(dab, ak, sizeof(double) * (km + 1), hipMemcpyHostToDevice, s));
+  FV3_HIP(hipMemcpyAsync(dab + km + 1, bk, sizeof(double) * (km + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL((k_convert<float, double>), grid_of_n(ndp), dim3(256), 0, s, ddp, dp1d, ndp);
+  FV3_HIP(hipGetLastError());
+  if (remap_only_f32(c, nq, dq, dqo, dpt, dp1d, dp2d, dab, dab + km + 1, ptop, kord_tr[0], kord_tm, t_min)) return 1;
+  hipLaunchKernelGGL((k_convert<double, float>), grid_of_n(ndp), dim3(256), 0, s, dp2d, ddp, ndp);
+  FV3_HIP(hipGetLastError());
+  FV3_HIP(hipMemcpyAsync(q, dqo, sizeof(float) * nqv, hipMemcpyDeviceToHost, s));
+  FV3_HIP(hipMemcpyAsync(delp, ddp, sizeof(float) * ndp, hipMemcpyDeviceToHost, s));
+  if (pt) FV3_HIP(hipMemcpyAsync(pt, dpt, sizeof(float) * ndp, hipMemcpyDeviceToHost, s));
+  FV3_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+static int tracer_common_r4(fv3_ctx *h, bool per_level, float *q, float *dp1, float *mfx, float *mfy,
+                            float *cx, float *cy, int npx, int npy, int npz, int nq, int hord, int q_split,
+                            int id_divg, int nord_tr, double trdm, int *nsplit_out) {
+  if (check_ctx(h, npx, npy, npz)) return 1;
+  Ctx &c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!q || !dp1 || !mfx || !mfy || !cx || !cy) return set_error("tracer_2d: NULL array argument");
+  if (nq < 1) return set_error("nq must be >= 1 (got %d)", nq);
+  if (q_split < 0) return set_error("q_split must be >= 0 (got %d)", q_split);
+  if (hord < 7 || hord > 13) return set_error("hord=%d: only the monotone orders 7..13 are implemented", hord);
+  if ((!per_level && trdm > 1.0e-4) || (!per_level && id_divg > 0) || nord_tr < 0)
+    return set_error("the 32-bit form has no deln_flux damping or id_divg (trdm=%g id_divg=%d)", trdm, id_divg);
+  return fused_transport_host<float>(c, per_level, q_split, q, dp1, mfx, mfy, cx, cy, nq, hord, nsplit_out);
 }
 
 // ---- row entry points (stateless) ---------------------------------------------

@@ -34,6 +34,12 @@ int fused_step(Ctx &c, bool fp32, int nq, const void *q_in, void *q_out, void *p
                int hord, int kord_tr, int kord_tm, double t_min, int nsplit_cap,
                const StepArena &arena = StepArena{nullptr, 0});
 
+// the FP32 column-shared remap alone: q_src -> q_dst (distinct buffers), pt
+// in place, delp_out the Eulerian thickness from the Lagrangian dp1
+int remap_only_f32(Ctx &c, int nq, const float *q_src, float *q_dst, float *pt, const double *dp1,
+                   double *delp_out, const double *ak, const double *bk, double ptop, int kord_tr,
+                   int kord_tm, double t_min);
+
 // tangent-linear / adjoint of the FP64 advection + tracer remap step
 int step_tl(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const double *dq0,
             double *q1, double *dq1, const double *dp1, double *delp_out, const double *cx,

@@ -53,7 +53,9 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
   if (hord < 7 || hord > 13) return set_error("hord=%d: only the monotone orders 7..13 are implemented", hord);
   if (c.graph_safe && nsplit_cap <= 0)
     return set_error("graph-safe steps need nsplit_cap > 0 (no device-to-host read)");
-  const bool fast_remap = std::is_same<T, float>::value && c.remap_mode == 0 && npz >= 8;
+  // the host-array entry points run the transport alone (c.transport_only)
+  const bool remap_phase = !c.transport_only;
+  const bool fast_remap = remap_phase && std::is_same<T, float>::value && c.remap_mode == 0 && npz >= 8;
   // the per-column remap synchronises the host with the stream, so only the
   // column-shared FP32 remap can be captured
   if (c.graph_safe && !fast_remap)
@@ -95,6 +97,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     FV3_HIP(hipStreamSynchronize(s));
     cap = *std::max_element(h.begin(), h.end());
     if (cap < 1) return set_error("nsplit < 1 from a non-finite Courant number");
+    c.last_nsplit = h;
   }
   // The FP32 form remaps out of place: the last transport pass of every
   // level lands in q_tmp (the caller's q_out serves as the alternate pass
@@ -273,7 +276,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
       FV3_HIP(hipStreamWaitEvent(s, c.prep_ev, 0));
     else if (remap_prep())
       return 1;
-  } else {
+  } else if (remap_phase) {
     FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, s));
   }
   if (timing && !pipe) FV3_HIP(hipEventRecord(c.ev[2], s));
@@ -286,7 +289,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
       }
     }
   }
-  if (!fast_remap) {
+  if (!fast_remap && remap_phase) {
     std::vector<int> kt(nq, kord_tr);
     if (remap_tracers_plain<T>(c, npz, nq, ak, bk, ptop, delp_out, q_out, pt, kt.data(), kord_tm,
                                t_min, 0))
@@ -302,6 +305,31 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     FV3_HIP(hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, s));
     FV3_HIP(hipStreamSynchronize(s));
     if (hf) return set_error("a level needs more sub-cycles than nsplit_cap=%d", nsplit_cap);
+  }
+  return 0;
+}
+
+int remap_only_f32(Ctx &c, int nq, const float *q_src, float *q_dst, float *pt, const double *dp1,
+                   double *delp_out, const double *ak, const double *bk, double ptop, int kord_tr,
+                   int kord_tm, double t_min) {
+  const Dims &d = c.d;
+  const int npz = c.npz, nt = c.ntiles, ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
+  hipStream_t s = c.stream;
+  if (remap_fast_check(npz, (long long)ni * nj, d.s2, kord_tr, kord_tm, pt != nullptr)) return 1;
+  const size_t b_pt = pt ? ((sizeof(float) * d.s2 * npz * nt) + 255) & ~(size_t)255 : 0;
+  const size_t b_col = remap_cols_bytes(npz, (long long)ni * nj);
+  char *sc = (char *)ctx_scratch(c, b_pt + b_col * nt);
+  if (!sc) return 1;
+  float *pt_src = pt ? (float *)sc : nullptr;
+  const long long tq = d.s2 * npz * nq, tl = d.s2 * npz;
+  FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * tl * nt, hipMemcpyDeviceToDevice, s));
+  if (pt) FV3_HIP(hipMemcpyAsync(pt_src, pt, sizeof(float) * tl * nt, hipMemcpyDeviceToDevice, s));
+  for (int t = 0; t < nt; t++) {
+    const RemapCols cols = remap_cols_carve(sc + b_pt + b_col * t, npz, (long long)ni * nj);
+    if (remap_fast_prep_f32(c, npz, ak, bk, ptop, dp1 + t * tl, delp_out + t * tl, cols, s)) return 1;
+    if (remap_fast_fields_f32(c, npz, nq, q_src + t * tq, q_dst + t * tq, pt_src ? pt_src + t * tl : nullptr,
+                              pt ? pt + t * tl : nullptr, kord_tr, kord_tm, t_min, cols, s))
+      return 1;
   }
   return 0;
 }

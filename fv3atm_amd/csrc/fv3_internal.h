@@ -110,6 +110,10 @@ struct Ctx {
   // on first use
   void *halo_tab[4] = {nullptr, nullptr, nullptr, nullptr};
   int halo_tab_n[4] = {0, 0, 0, 0};
+  // the fused step without its remap (the host-array transport entry points)
+  // and the sub-cycle counts of the last step that derived them on the host
+  bool transport_only = false;
+  std::vector<int> last_nsplit;
   // Lagrangian_to_Eulerian buffers that outlive the column-kernel calls
   void *l2e_buf = nullptr;
   size_t l2e_bytes = 0;

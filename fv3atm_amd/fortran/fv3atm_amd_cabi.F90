@@ -156,6 +156,88 @@ module fv3atm_amd_cabi
        integer(c_int), intent(out) :: passes
      end function fv3_step_timing
 
+     integer(c_int) function fv3_tracer_2d_1l_r4(ctx, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, &
+          hord, q_split, dt, id_divg, nord_tr, trdm, lim_fac, nsplit_out) bind(C, name="fv3_tracer_2d_1l_r4")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, q, dp1, mfx, mfy, cx, cy, nsplit_out
+       integer(c_int), value :: npx, npy, npz, nq, hord, q_split, id_divg, nord_tr
+       real(c_double), value :: dt, trdm, lim_fac
+     end function fv3_tracer_2d_1l_r4
+
+     integer(c_int) function fv3_tracer_2d_r4(ctx, q, dp1, mfx, mfy, cx, cy, npx, npy, npz, nq, &
+          hord, q_split, dt, id_divg, nord_tr, trdm, lim_fac, nsplit_out) bind(C, name="fv3_tracer_2d_r4")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, q, dp1, mfx, mfy, cx, cy, nsplit_out
+       integer(c_int), value :: npx, npy, npz, nq, hord, q_split, id_divg, nord_tr
+       real(c_double), value :: dt, trdm, lim_fac
+     end function fv3_tracer_2d_r4
+
+     integer(c_int) function fv3_lagrangian_to_eulerian_tracers_r4(ctx, km, nq, ak, bk, ptop, delp, q, &
+          pt, kord_tr, kord_tm, t_min, fill) bind(C, name="fv3_lagrangian_to_eulerian_tracers_r4")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, ak, bk, delp, q, pt, kord_tr
+       integer(c_int), value :: km, nq, kord_tm, fill
+       real(c_double), value :: ptop, t_min
+     end function fv3_lagrangian_to_eulerian_tracers_r4
+
+     integer(c_int) function fv3_step_dev_noalloc(ctx, form, nq, q_in, q_out, pt, dp1, delp_out, cx, cy, &
+          mfx, mfy, ak, bk, ptop, hord, kord_tr, kord_tm, t_min, nsplit_cap, stream) &
+          bind(C, name="fv3_step_dev_noalloc")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, q_in, q_out, pt, dp1, delp_out, cx, cy, mfx, mfy, ak, bk, stream
+       integer(c_int), value :: form, nq, hord, kord_tr, kord_tm, nsplit_cap
+       real(c_double), value :: ptop, t_min
+     end function fv3_step_dev_noalloc
+
+     integer(c_int) function fv3_ctx_set_graph_safe(ctx, on) bind(C, name="fv3_ctx_set_graph_safe")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: on
+     end function fv3_ctx_set_graph_safe
+
+     integer(c_int) function fv3_step_status(ctx, nsplit_overflow) bind(C, name="fv3_step_status")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), intent(out) :: nsplit_overflow
+     end function fv3_step_status
+
+     integer(c_int) function fv3_step_tl_dev(ctx, per_level, q_split, nq, q0, dq0, q1, dq1, dp1, &
+          delp_out, cx, cy, mfx, mfy, ak, bk, ptop, hord, kord_tr, stream) bind(C, name="fv3_step_tl_dev")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, q0, dq0, q1, dq1, dp1, delp_out, cx, cy, mfx, mfy, ak, bk, stream
+       integer(c_int), value :: per_level, q_split, nq, hord, kord_tr
+       real(c_double), value :: ptop
+     end function fv3_step_tl_dev
+
+     integer(c_int) function fv3_step_ad_dev(ctx, per_level, q_split, nq, q0, qbar1, qbar0, dp1, cx, &
+          cy, mfx, mfy, ak, bk, ptop, hord, kord_tr, stream) bind(C, name="fv3_step_ad_dev")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, q0, qbar1, qbar0, dp1, cx, cy, mfx, mfy, ak, bk, stream
+       integer(c_int), value :: per_level, q_split, nq, hord, kord_tr
+       real(c_double), value :: ptop
+     end function fv3_step_ad_dev
+
+     integer(c_int) function fv3_halo_update(ctx, field, nslab, elem_bytes, stream) bind(C, name="fv3_halo_update")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, field, stream
+       integer(c_int), value :: nslab, elem_bytes
+     end function fv3_halo_update
+
+     integer(c_int) function fv3_halo_update_vector(ctx, u, v, nslab, grid, elem_bytes, stream) &
+          bind(C, name="fv3_halo_update_vector")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, u, v, stream
+       integer(c_int), value :: nslab, grid, elem_bytes
+     end function fv3_halo_update_vector
+
+     integer(c_int) function fv3_kmp(npz, p_ref, ak, bk, kmp) bind(C, name="fv3_kmp")
+       import :: c_int, c_ptr, c_double
+       integer(c_int), value :: npz
+       real(c_double), value :: p_ref
+       type(c_ptr), value :: ak, bk
+       integer(c_int), intent(out) :: kmp
+     end function fv3_kmp
+
      function c_strlen(s) bind(C, name="strlen")
        import :: c_ptr, c_size_t
        type(c_ptr), value :: s
@@ -169,11 +251,18 @@ module fv3atm_amd_cabi
   public :: fv3_ctx_set_timing, fv3_ctx_set_launch, fv3_ctx_set_concurrency, fv3_ctx_set_transport, &
             fv3_ctx_set_pipeline
   public :: fv3_step_timing
+  public :: fv3_tracer_2d_1l_r4, fv3_tracer_2d_r4, fv3_lagrangian_to_eulerian_tracers_r4
+  public :: fv3_step_dev_noalloc, fv3_ctx_set_graph_safe, fv3_step_status, fv3_step_tl_dev, &
+            fv3_step_ad_dev, fv3_halo_update, fv3_halo_update_vector, fv3_kmp
 
-  ! one cached context per (bounds, npx, npy, npz); the metrics are uploaded
-  ! when it is created
-  type(c_ptr), save :: ctx_cached = c_null_ptr
-  integer, save :: key(11) = -huge(1)
+  ! Device contexts, one per (bounds, npx, npy, npz): fv_tp_2d (npz = 1) and
+  ! tracer_2d (npz levels) alternate without re-creating them.  The metrics
+  ! are uploaded on every call, so a gridstruct recomputed in place (a moving
+  ! nest, fv_moving_nest.F90:1607-1613) is always the one the kernels see.
+  integer, parameter :: NCTX = 4
+  type(c_ptr), save :: ctx_cached(NCTX) = c_null_ptr
+  integer, save :: key(11, NCTX) = -huge(1)
+  integer, save :: ctx_next = 1
 
 contains
 
@@ -204,24 +293,30 @@ contains
     end if
   end subroutine fv3_check
 
-  ! The device context for one tile of gridstruct (created and filled with the
-  ! metrics on first use, re-created when the shape changes).
+  ! The device context for one tile of gridstruct: found (or created) by its
+  ! shape, then given the current metrics.
   function fv3_context(gridstruct, bd, npx, npy, npz) result(ctx)
     type(fv_grid_type), intent(in), target :: gridstruct
     type(fv_grid_bounds_type), intent(in) :: bd
     integer, intent(in) :: npx, npy, npz
     type(c_ptr) :: ctx
-    integer :: k(11)
+    integer :: k(11), slot, i
     type(fv3_bounds_c) :: b
     type(fv3_grid_c) :: g
     k = [bd%is, bd%ie, bd%js, bd%je, bd%isd, bd%ied, bd%jsd, bd%jed, npx, npy, npz]
-    if (c_associated(ctx_cached) .and. all(k == key)) then
-       ctx = ctx_cached
-       return
+    slot = 0
+    do i = 1, NCTX
+       if (c_associated(ctx_cached(i)) .and. all(k == key(:, i))) slot = i
+    end do
+    if (slot == 0) then
+       slot = ctx_next
+       ctx_next = mod(ctx_next, NCTX) + 1
+       if (c_associated(ctx_cached(slot))) &
+            call fv3_check(fv3_ctx_destroy(ctx_cached(slot)), 'fv3_ctx_destroy')
+       b = fv3_bounds_c(bd%is, bd%ie, bd%js, bd%je, bd%isd, bd%ied, bd%jsd, bd%jed)
+       call fv3_check(fv3_ctx_create(b, npx, npy, npz, 1, ctx_cached(slot)), 'fv3_ctx_create')
+       key(:, slot) = k
     end if
-    call fv3_context_release()
-    b = fv3_bounds_c(bd%is, bd%ie, bd%js, bd%je, bd%isd, bd%ied, bd%jsd, bd%jed)
-    call fv3_check(fv3_ctx_create(b, npx, npy, npz, 1, ctx_cached), 'fv3_ctx_create')
     g%area = c_loc(gridstruct%area)
     g%rarea = c_loc(gridstruct%rarea)
     g%dx = c_loc(gridstruct%dx)
@@ -240,15 +335,18 @@ contains
     g%ne_corner = merge(1, 0, gridstruct%ne_corner)
     g%bounded_domain = merge(1, 0, gridstruct%bounded_domain)
     g%grid_type = gridstruct%grid_type
-    call fv3_check(fv3_ctx_set_grid(ctx_cached, 1, g), 'fv3_ctx_set_grid')
-    key = k
-    ctx = ctx_cached
+    call fv3_check(fv3_ctx_set_grid(ctx_cached(slot), 1, g), 'fv3_ctx_set_grid')
+    ctx = ctx_cached(slot)
   end function fv3_context
 
   subroutine fv3_context_release()
-    if (c_associated(ctx_cached)) call fv3_check(fv3_ctx_destroy(ctx_cached), 'fv3_ctx_destroy')
-    ctx_cached = c_null_ptr
-    key = -huge(1)
+    integer :: i
+    do i = 1, NCTX
+       if (c_associated(ctx_cached(i))) call fv3_check(fv3_ctx_destroy(ctx_cached(i)), 'fv3_ctx_destroy')
+       ctx_cached(i) = c_null_ptr
+       key(:, i) = -huge(1)
+    end do
+    ctx_next = 1
   end subroutine fv3_context_release
 
 end module fv3atm_amd_cabi

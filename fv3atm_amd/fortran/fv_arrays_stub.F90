@@ -23,8 +23,14 @@ module fv_arrays_mod
   implicit none
   private
   integer, public, parameter :: R_GRID = selected_real_kind(12, 100)
-  ! kind of the dycore's default real on this path (the 8-byte build)
+  ! kind of the dycore's default real on this path: 8 bytes, or 4 in the
+  ! 32BIT build (OVERLOAD_R4, ci/CMakeLists.txt:20-22,35,59-63); the grid
+  ! metrics stay R_GRID either way
+#ifdef OVERLOAD_R4
+  integer, public, parameter :: FVPRC = selected_real_kind(6, 37)
+#else
   integer, public, parameter :: FVPRC = R_GRID
+#endif
 
   type, public :: fv_grid_bounds_type
      integer :: is, ie, js, je

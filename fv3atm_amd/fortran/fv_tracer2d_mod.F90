@@ -7,12 +7,19 @@
 ! Semantics as upstream: q, dp1 updated in place, cx/cy/mfx/mfy scaled by
 ! 1/nsplit in place.  domain, q_pack and dp1_pack are FMS halo handles the
 ! device path does not need: the halo of a single tile stays as passed in.
+! Built with OVERLOAD_R4 (the 32-bit dycore) the same routines take 4-byte
+! arrays and run the FP32 fused kernels (fv3_tracer_2d_1l_r4 / _r4).
 
 module fv_tracer2d_mod
   use iso_c_binding
   use fv_arrays_mod, only: fv_grid_bounds_type, fv_grid_type, FVPRC
   use mpp_domains_mod, only: domain2d, group_halo_update_type
+#ifdef OVERLOAD_R4
+  use fv3atm_amd_cabi, only: fv3_check, fv3_context, fv3_tracer_2d_1l => fv3_tracer_2d_1l_r4, &
+       fv3_tracer_2d => fv3_tracer_2d_r4
+#else
   use fv3atm_amd_cabi, only: fv3_check, fv3_context, fv3_tracer_2d_1l, fv3_tracer_2d
+#endif
   implicit none
   private
   public :: tracer_2d_1L, tracer_2d

@@ -4,8 +4,14 @@
 !   (fv_mapz_mod lagrangian_to_eulerian_tracers)
 ! the way fv_dynamics sequences them, and writes the results.
 !
-!   fv3_harness IN OUT      run the step
-!   fv3_harness --host      argument checks that fail before any device work
+!   fv3_harness IN OUT            run the step
+!   fv3_harness IN OUT --regrid   run it once, then change the metrics in place
+!                                 (area x 1.1, rarea = 1/area, dxa x 1.05) with
+!                                 the same bounds and run it again from the
+!                                 same state: the output is the second run's
+!   fv3_harness --host            argument checks that fail before any device work
+! Built with OVERLOAD_R4 (fv3_harness_r4) the arrays are the 32-bit build's
+! 4-byte reals; the files stay float64.
 !
 ! IN (little-endian stream): int32 n, npz, nq, hord, kord_tr, kord_tm, fill,
 ! sw, se, ne, nw, grid_type; float64 ptop, t_min, dt, trdm, lim_fac, da_min;
@@ -24,7 +30,7 @@ program fv3_harness
   use fv_tracer2d_mod, only: tracer_2d_1L, last_nsplit
   use fv_mapz_mod, only: lagrangian_to_eulerian_tracers
   implicit none
-  character(len=4096) :: fin, fout
+  character(len=4096) :: fin, fout, opt
   integer :: hdr(12), n, npz, nq, hord, kord_tr, kord_tm, nd, u
   real(R_GRID) :: rhdr(6)
   type(fv_grid_bounds_type) :: bd
@@ -33,6 +39,9 @@ program fv3_harness
   type(group_halo_update_type) :: q_pack, dp1_pack
   real(FVPRC), allocatable, target :: q(:,:,:,:), dp1(:,:,:), mfx(:,:,:), mfy(:,:,:), cx(:,:,:), &
        cy(:,:,:), ak(:), bk(:), pt(:,:,:), delp(:,:,:)
+  ! the inputs as read (float64)
+  real(R_GRID), allocatable :: q8(:,:,:,:), dp18(:,:,:), mfx8(:,:,:), mfy8(:,:,:), cx8(:,:,:), &
+       cy8(:,:,:), ak8(:), bk8(:), pt8(:,:,:)
   integer, allocatable :: kord(:)
 
   call get_command_argument(1, fin)
@@ -41,6 +50,7 @@ program fv3_harness
      stop
   end if
   call get_command_argument(2, fout)
+  call get_command_argument(3, opt)
 
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) hdr
@@ -58,27 +68,49 @@ program fv3_harness
   gs%nw_corner = hdr(11) /= 0
   gs%grid_type = hdr(12)
   gs%da_min = rhdr(6)
-  allocate(q(-2:n+3, -2:n+3, npz, nq), dp1(-2:n+3, -2:n+3, npz), mfx(1:n+1, 1:n, npz), &
-       mfy(1:n, 1:n+1, npz), cx(1:n+1, -2:n+3, npz), cy(-2:n+3, 1:n+1, npz), ak(npz+1), bk(npz+1), &
-       pt(-2:n+3, -2:n+3, npz))
-  read(u) q, dp1, mfx, mfy, cx, cy, ak, bk, pt
+  allocate(q8(-2:n+3, -2:n+3, npz, nq), dp18(-2:n+3, -2:n+3, npz), mfx8(1:n+1, 1:n, npz), &
+       mfy8(1:n, 1:n+1, npz), cx8(1:n+1, -2:n+3, npz), cy8(-2:n+3, 1:n+1, npz), ak8(npz+1), bk8(npz+1), &
+       pt8(-2:n+3, -2:n+3, npz))
+  read(u) q8, dp18, mfx8, mfy8, cx8, cy8, ak8, bk8, pt8
   close(u)
-
-  ! the remap source is the Lagrangian thickness the step starts from
-  delp = dp1
-  call tracer_2d_1L(q, dp1, mfx, mfy, cx, cy, gs, bd, domain, n + 1, n + 1, npz, nq, hord, 0, &
-       rhdr(3), 0, q_pack, dp1_pack, 0, rhdr(4), rhdr(5))
   allocate(kord(nq))
   kord = kord_tr
-  call lagrangian_to_eulerian_tracers(npz, nq, ak, bk, rhdr(1), delp, q, pt, kord, kord_tm, rhdr(2), &
-       hdr(7) /= 0, gs, bd, n + 1, n + 1)
+
+  call run_step()
+  if (trim(opt) == '--regrid') then
+     ! a gridstruct recomputed in place with the same bounds (a moving nest)
+     gs%area = gs%area * 1.1_R_GRID
+     gs%rarea = 1.0_R_GRID / gs%area
+     gs%dxa = gs%dxa * 1.05_R_GRID
+     call run_step()
+  end if
   call fv3_context_release()
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write(u) q, dp1, delp, pt, int(last_nsplit, c_int32_t)
+  write(u) real(q, R_GRID), real(dp1, R_GRID), real(delp, R_GRID), real(pt, R_GRID), &
+       int(last_nsplit, c_int32_t)
   close(u)
 
 contains
+
+  ! tracer_2d_1L, then the tracer/pt remap from the Lagrangian thickness the
+  ! step starts from, as fv_dynamics sequences them
+  subroutine run_step()
+    q = real(q8, FVPRC)
+    dp1 = real(dp18, FVPRC)
+    mfx = real(mfx8, FVPRC)
+    mfy = real(mfy8, FVPRC)
+    cx = real(cx8, FVPRC)
+    cy = real(cy8, FVPRC)
+    ak = real(ak8, FVPRC)
+    bk = real(bk8, FVPRC)
+    pt = real(pt8, FVPRC)
+    delp = dp1
+    call tracer_2d_1L(q, dp1, mfx, mfy, cx, cy, gs, bd, domain, n + 1, n + 1, npz, nq, hord, 0, &
+         real(rhdr(3), FVPRC), 0, q_pack, dp1_pack, 0, real(rhdr(4), FVPRC), real(rhdr(5), FVPRC))
+    call lagrangian_to_eulerian_tracers(npz, nq, ak, bk, real(rhdr(1), FVPRC), delp, q, pt, kord, &
+         kord_tm, real(rhdr(2), FVPRC), hdr(7) /= 0, gs, bd, n + 1, n + 1)
+  end subroutine run_step
 
   subroutine host_checks()
     real(c_double), target :: a(64)

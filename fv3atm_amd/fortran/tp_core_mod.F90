@@ -53,9 +53,37 @@ contains
        damp_cc = damp_c
        p_damp = c_loc(damp_cc)
     end if
+#ifdef OVERLOAD_R4
+    block
+      ! fv_tp_2d computes in FP64: 8-byte copies of the field and its inputs
+      real(c_double), allocatable, target :: q8(:,:), crx8(:,:), cry8(:,:), fx8(:,:), fy8(:,:), &
+           xfx8(:,:), yfx8(:,:), rax8(:,:), ray8(:,:), mfx8(:,:), mfy8(:,:), mass8(:,:)
+      q8 = q; crx8 = crx; cry8 = cry; xfx8 = xfx; yfx8 = yfx; rax8 = ra_x; ray8 = ra_y
+      allocate(fx8(size(fx, 1), size(fx, 2)), fy8(size(fy, 1), size(fy, 2)))
+      if (present(mfx)) then
+         mfx8 = mfx
+         p_mfx = c_loc(mfx8)
+      end if
+      if (present(mfy)) then
+         mfy8 = mfy
+         p_mfy = c_loc(mfy8)
+      end if
+      if (present(mass)) then
+         mass8 = mass
+         p_mass = c_loc(mass8)
+      end if
+      call fv3_check(fv3_fv_tp_2d(ctx, c_loc(q8), c_loc(crx8), c_loc(cry8), npx, npy, hord, c_loc(fx8), &
+           c_loc(fy8), c_loc(xfx8), c_loc(yfx8), c_loc(rax8), c_loc(ray8), real(lim_fac, c_double), &
+           p_mfx, p_mfy, p_mass, p_nord, p_damp), 'fv_tp_2d')
+      q = real(q8, FVPRC)
+      fx = real(fx8, FVPRC)
+      fy = real(fy8, FVPRC)
+    end block
+#else
     call fv3_check(fv3_fv_tp_2d(ctx, c_loc(q), c_loc(crx), c_loc(cry), npx, npy, hord, c_loc(fx), &
          c_loc(fy), c_loc(xfx), c_loc(yfx), c_loc(ra_x), c_loc(ra_y), real(lim_fac, c_double), &
          p_mfx, p_mfy, p_mass, p_nord, p_damp), 'fv_tp_2d')
+#endif
   end subroutine fv_tp_2d
 
 end module tp_core_mod

@@ -68,7 +68,11 @@ int fv3_fv_tp_2d(fv3_ctx *ctx, double *q, const double *crx, const double *cry, 
 
 /* fv_tracer2d.F90 tracer_2d_1L(q, dp1, mfx, mfy, cx, cy, gridstruct, bd,
  *   domain, npx, npy, npz, nq, hord, q_split, dt, id_divg, q_pack, dp1_pack,
- *   nord_tr, trdm, lim_fac): per-level sub-cycling, plain FP64 form.
+ *   nord_tr, trdm, lim_fac): per-level sub-cycling, FP64, by the fused
+ *   transport kernels (bit for bit the staged form; the staged form runs
+ *   when tracer_2d asks for deln_flux damping or id_divg, or runs on a
+ *   context of several faces).  q's cube-corner halo blocks come back as
+ *   upstream's copy_corners leaves them.
  * All arrays inout as upstream (cx, cy, mfx, mfy scaled by 1/nsplit(k)).
  * nsplit_out (npz, may be NULL) receives the sub-cycle counts. */
 int fv3_tracer_2d_1l(fv3_ctx *ctx, double *q, double *dp1, double *mfx, double *mfy,
@@ -81,6 +85,21 @@ int fv3_tracer_2d(fv3_ctx *ctx, double *q, double *dp1, double *mfx, double *mfy
                   double *cy, int npx, int npy, int npz, int nq, int hord, int q_split,
                   double dt, int id_divg, int nord_tr, double trdm, double lim_fac,
                   int *nsplit_out);
+
+/* The same two routines for the dycore's 32-bit build (kind_dyn = 4: the
+ * CMake 32BIT / OVERLOAD_R4 build, ci/CMakeLists.txt:20-22,35,59-63): float
+ * arrays, computed by the FP32 fused transport (FV3_FORM_FUSED32 kernels)
+ * with the in-place semantics above.  deln_flux damping (trdm > 1e-4) and
+ * id_divg are not part of this form and fail.  On a context of several
+ * faces, tracer_2d_r4 leaves q's halo strips as they came in (upstream
+ * leaves the exchange made before its last sub-cycle there). */
+int fv3_tracer_2d_1l_r4(fv3_ctx *ctx, float *q, float *dp1, float *mfx, float *mfy, float *cx,
+                        float *cy, int npx, int npy, int npz, int nq, int hord, int q_split,
+                        double dt, int id_divg, int nord_tr, double trdm, double lim_fac,
+                        int *nsplit_out);
+int fv3_tracer_2d_r4(fv3_ctx *ctx, float *q, float *dp1, float *mfx, float *mfy, float *cx,
+                     float *cy, int npx, int npy, int npz, int nq, int hord, int q_split, double dt,
+                     int id_divg, int nord_tr, double trdm, double lim_fac, int *nsplit_out);
 
 /* fv_mapz.F90 map1_ppm(km, pe1, q1, qs, kn, pe2, q2, i1, i2, j, ibeg, iend,
  *   jbeg, jend, iv, kord): one row j, columns i1:i2; pe1 (i1:i2, km+1),
@@ -113,6 +132,14 @@ int fv3_lagrangian_to_eulerian_tracers(fv3_ctx *ctx, int km, int nq, const doubl
                                        const double *bk, double ptop, double *delp, double *q,
                                        double *pt, const int *kord_tr, int kord_tm,
                                        double t_min, int fill);
+
+/* The tracer/pt subset above for the 32-bit build: float delp, q, pt (ak, bk
+ * double), by the FP32 column-shared remap (FP64 pressures); one kord_tr for
+ * every tracer, km >= 8, no fillz. */
+int fv3_lagrangian_to_eulerian_tracers_r4(fv3_ctx *ctx, int km, int nq, const double *ak,
+                                          const double *bk, double ptop, float *delp, float *q,
+                                          float *pt, const int *kord_tr, int kord_tm,
+                                          double t_min, int fill);
 
 /* Device-resident advection + remap step (no counterpart upstream as one call:
  * it is tracer_2d_1L followed by the tracer/pt subset of

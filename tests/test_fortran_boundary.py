@@ -80,3 +80,59 @@ def test_fortran_step_matches_oracle(gpu_ready, tmp_path):
     assert_close(interior(got["q"]), interior(q), 1e-12, "q", floor=1e-300)
     assert_close(interior(got["delp"]), interior(delp), 1e-12, "delp")
     assert_close(interior(got["pt"]), interior(pt), 1e-12, "pt")
+
+
+HARNESS_R4 = os.path.join(ROOT, "fv3atm_amd", "fv3_harness_r4")
+
+
+@needs_harness
+@pytest.mark.gpu
+def test_fortran_32bit_build_matches_oracle(gpu_ready, tmp_path):
+    # the dycore's 32-bit build (OVERLOAD_R4: 4-byte q, dp1, winds, delp, pt)
+    # through the same Fortran interfaces runs the FP32 fused transport and
+    # the FP32 column-shared remap: within 1e-5 relative L2 of the FP64 oracle
+    from oracle import oracle
+    from tests._util import interior, rel_l2
+    assert os.path.exists(HARNESS_R4)
+    case = cases.make_case("C20L9Q6jet", n=20, npz=9, nq=6, jet=True, courant=2.3, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=12, with_pt=True)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_input(fin, case)
+    print("fortran 32-bit harness C20L9Q6jet", flush=True)
+    r = subprocess.run([HARNESS_R4, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_output(fout, case)
+    adv = oracle.tracer_2d(case, "1L")
+    q, delp, pt = oracle.remap_tracers(case, adv["q"], case.dp1, case.pt)
+    np.testing.assert_array_equal(got["nsplit"], adv["nsplit"])
+    for iq in range(case.nq):
+        e = rel_l2(interior(got["q"][..., iq]), interior(q[..., iq]))
+        print(f"tracer {iq}: rel L2 {e:.2e}")
+        assert e <= 1e-5, (iq, e)
+    assert rel_l2(interior(got["pt"]), interior(pt)) <= 1e-6
+    assert rel_l2(interior(got["dp1"]), interior(adv["dp1"])) <= 1e-6
+    assert rel_l2(interior(got["delp"]), interior(delp)) <= 1e-6
+
+
+@needs_harness
+@pytest.mark.gpu
+def test_fortran_metrics_recomputed_in_place(gpu_ready, tmp_path):
+    # a gridstruct changed in place with the same bounds (a moving nest):
+    # the second call must use the new metrics
+    from oracle import oracle
+    from tests._util import assert_close, interior
+    case = cases.make_case("C16L8Q2", n=16, npz=8, nq=2, courant=0.8, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=13, with_pt=True)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_input(fin, case)
+    r = subprocess.run([HARNESS, fin, fout, "--regrid"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_output(fout, case)
+    g = case.grids[0]
+    g.area = np.asfortranarray(g.area * 1.1)
+    g.rarea = np.asfortranarray(1.0 / g.area)
+    g.dxa = np.asfortranarray(g.dxa * 1.05)
+    adv = oracle.tracer_2d(case, "1L")
+    q, delp, pt = oracle.remap_tracers(case, adv["q"], case.dp1, case.pt)
+    assert_close(interior(got["q"]), interior(q), 1e-12, "q on the new metrics", floor=1e-300)
+    assert_close(interior(got["dp1"]), interior(adv["dp1"]), 1e-12, "dp1 on the new metrics")

@@ -49,14 +49,21 @@ def test_tracer_2d_1L_c48(gpu_ready, hord):
     _check_all(a, ns, ref, f"C48 hord{hord}")
 
 
-def test_tracer_2d_1L_subcycled(gpu_ready):
-    # jet profile: uneven nsplit(k) in 1..3, frozen single-tile halos between sub-cycles
+@pytest.mark.parametrize("variant", ["1L", "2d"])
+def test_tracer_2d_1L_subcycled(gpu_ready, variant):
+    # jet profile: uneven nsplit(k) in 1..3 (one count of 3 for tracer_2d),
+    # frozen single-tile halos between sub-cycles; every array is compared
+    # whole, so q's cube-corner halo blocks must come back as copy_corners
+    # leaves them on the levels transported in place
     case = cases.make_case("C24L12Q3jet", n=24, npz=12, nq=3, jet=True, courant=2.5,
                            delp_kind=1, delp_amp=(0.05, 0.2), seed=3)
-    a, ns, ref = _run_tracer(case, "1L")
+    a, ns, ref = _run_tracer(case, variant)
     print("nsplit", ns)
-    assert ns.max() >= 2 and ns.min() == 1
-    _check_all(a, ns, ref, "C24 jet")
+    if variant == "1L":
+        assert ns.max() >= 2 and ns.min() == 1
+    else:
+        assert (ns == 3).all()
+    _check_all(a, ns, ref, f"C24 jet {variant}")
 
 
 @pytest.mark.parametrize("nord,trdm", [(0, 0.0), (0, 0.12), (1, 0.1), (2, 0.08)])
