@@ -305,6 +305,94 @@ def halo_update_vector(ctx: Context, u, v, nslab, grid, stream=None):
                                             _stream(stream)), "fv3_halo_update_vector")
 
 
+def _dev_field(name, t, need):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()):
+        raise TypeError(f"{name}: a contiguous torch tensor on the HIP device is required")
+    if t.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"{name}: float32 or float64 required, got {t.dtype}")
+    if t.numel() < need:
+        raise ValueError(f"{name}: {need} elements required, got {t.numel()}")
+
+
+def _slab(ctx: Context):
+    b = ctx.bd
+    return (b.ied - b.isd + 1) * (b.jed - b.jsd + 1)
+
+
+def field_checksum(ctx: Context, field, nslab, stream=None):
+    """FMS mpp_chksum of each tile's compute domain of a device field of
+    nslab (isd:ied, jsd:jed) slabs per tile: the sum modulo 2**64 of the
+    values' bit patterns as two's-complement integers.  Returns ntiles ints."""
+    _dev_field("field", field, ctx.ntiles * nslab * _slab(ctx))
+    out = (ctypes.c_ulonglong * ctx.ntiles)()
+    check(_ffi.lib().fv3_field_checksum(ctx.handle, ptr(field), nslab, field.element_size(), out,
+                                        _stream(stream)), "fv3_field_checksum")
+    return [int(v) for v in out]
+
+
+def _names(names):
+    arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+    return arr
+
+
+def tracer_restart_write(ctx: Context, prefix, names, q, tile_base=1, stream=None):
+    """Writes <prefix>.tile<t>.nc (fv_tracer.res layout, one file per tile)
+    from the device tracers q ((isd:ied, jsd:jed, npz, nq) per tile)."""
+    _dev_field("q", q, ctx.ntiles * ctx.npz * len(names) * _slab(ctx))
+    check(_ffi.lib().fv3_tracer_restart_write(ctx.handle, prefix.encode(), tile_base, len(names), _names(names),
+                                              ptr(q), q.element_size(), _stream(stream)),
+          "fv3_tracer_restart_write")
+
+
+def tracer_restart_read(ctx: Context, prefix, names, q, tile_base=1, ignore_cksum=False, stream=None):
+    """Reads the tracers named in names from <prefix>.tile<t>.nc into the
+    compute domain of the device array q, checking each FMS checksum unless
+    ignore_cksum."""
+    _dev_field("q", q, ctx.ntiles * ctx.npz * len(names) * _slab(ctx))
+    check(_ffi.lib().fv3_tracer_restart_read(ctx.handle, prefix.encode(), tile_base, len(names), _names(names),
+                                             ptr(q), q.element_size(), int(bool(ignore_cksum)), _stream(stream)),
+          "fv3_tracer_restart_read")
+
+
+def history_write(ctx: Context, path, fields, pfull, phalf, *, out_bytes=4, quantize_nsd=0, tile_base=1,
+                  lon=None, lat=None, ak=None, bk=None, time=0.0, time_units="hours since 1970-01-01 00:00:00",
+                  time_iso="1970-01-01T00:00:00Z", stream=None):
+    """Writes the history bundle file path.  fields: (name, long_name, units,
+    device tensor, nlev) with nlev 1 or npz; pfull, phalf: host float arrays;
+    lon, lat: host float64 (tile, j, i) arrays of the compute domain or None."""
+    import numpy as np
+    fs = (_ffi.HistoryField * max(1, len(fields)))()
+    keep = []
+    for i, (name, long_name, units, t, nlev) in enumerate(fields):
+        _dev_field(name, t, ctx.ntiles * nlev * _slab(ctx))
+        fs[i] = _ffi.HistoryField(name.encode(), long_name.encode(), units.encode(), t.data_ptr(), nlev,
+                                  t.element_size())
+    pf = np.ascontiguousarray(pfull, dtype=np.float32)
+    ph = np.ascontiguousarray(phalf, dtype=np.float32)
+    if pf.size != ctx.npz or ph.size != ctx.npz + 1:
+        raise ValueError("pfull needs npz values and phalf npz + 1")
+
+    def dp(a, n):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != n:
+            raise ValueError(f"expected {n} values, got {a.size}")
+        keep.append(a)
+        return a.ctypes.data_as(_ffi.DP)
+
+    b = ctx.bd
+    ncell = ctx.ntiles * (b.ie - b.is_ + 1) * (b.je - b.js + 1)
+    args = _ffi.HistoryArgs(len(fields), fs, out_bytes, quantize_nsd, tile_base,
+                            pf.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                            ph.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                            dp(lon, ncell), dp(lat, ncell), dp(ak, ctx.npz + 1), dp(bk, ctx.npz + 1),
+                            time, time_units.encode(), time_iso.encode())
+    check(_ffi.lib().fv3_history_write(ctx.handle, path.encode(), ctypes.byref(args), _stream(stream)),
+          "fv3_history_write")
+
+
 def lagrangian_to_eulerian_dev(ctx: Context, form, km, nq, ak, bk, ptop, state, kord_tr=9, kord_tm=10,
                                kord_mt=9, kord_wz=9, hydrostatic=False, consv=0.0, t_min=184.0,
                                fill=False, akap=287.05 / 1004.6, cp=1004.6, rdgas=287.05, grav=9.80665,

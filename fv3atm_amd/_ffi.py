@@ -42,6 +42,19 @@ class L2EState(ctypes.Structure):
 FAST_PHYSICS_FN = ctypes.CFUNCTYPE(ctypes.c_int, P, ctypes.POINTER(L2EState))
 
 
+class HistoryField(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("long_name", ctypes.c_char_p), ("units", ctypes.c_char_p),
+                ("data", P), ("nlev", I), ("elem_bytes", I)]
+
+
+class HistoryArgs(ctypes.Structure):
+    _fields_ = [("nfields", I), ("fields", ctypes.POINTER(HistoryField)), ("out_bytes", I),
+                ("quantize_nsd", I), ("tile_base", I), ("pfull", ctypes.POINTER(ctypes.c_float)),
+                ("phalf", ctypes.POINTER(ctypes.c_float))] + \
+               [(k, DP) for k in ("lon", "lat", "ak", "bk")] + \
+               [("time", D), ("time_units", ctypes.c_char_p), ("time_iso", ctypes.c_char_p)]
+
+
 class L2EArgs(ctypes.Structure):
     _fields_ = [("km", I), ("nq", I), ("ak", P), ("bk", P)] + \
                [(k, D) for k in ("ptop", "akap", "cp", "rdgas", "grav", "consv", "t_min", "p_ref", "mdt")] + \
@@ -82,6 +95,10 @@ SIGNATURES = {
     "fv3_halo_update_vector": (I, [P, P, P, I, I, I, P]),
     "fv3_lagrangian_to_eulerian_dev": (I, [P, I, ctypes.POINTER(L2EArgs), P]),
     "fv3_kmp": (I, [I, D, P, P, IP]),
+    "fv3_field_checksum": (I, [P, P, I, I, ctypes.POINTER(ctypes.c_ulonglong), P]),
+    "fv3_tracer_restart_write": (I, [P, ctypes.c_char_p, I, I, ctypes.POINTER(ctypes.c_char_p), P, I, P]),
+    "fv3_tracer_restart_read": (I, [P, ctypes.c_char_p, I, I, ctypes.POINTER(ctypes.c_char_p), P, I, I, P]),
+    "fv3_history_write": (I, [P, ctypes.c_char_p, ctypes.POINTER(HistoryArgs), P]),
     "fv3_step_tl_dev": (I, [P, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, D, I, I, P]),
     "fv3_step_ad_dev": (I, [P, I, I, I, P, P, P, P, P, P, P, P, P, P, D, I, I, P]),
 }

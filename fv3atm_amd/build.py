@@ -36,6 +36,7 @@ HIP_SOURCES = {
     "noalloc.hip": ["-ffp-contract=off"],
     "halo.hip": ["-ffp-contract=off"],
     "l2e.hip": ["-ffp-contract=off"],
+    "io.hip": ["-ffp-contract=off"],
 }
 
 

@@ -301,6 +301,65 @@ int fv3_lagrangian_to_eulerian_dev(fv3_ctx *ctx, int form, const fv3_l2e_args *a
  * level whose full-level pressure at p_ref exceeds 10 hPa (host arrays). */
 int fv3_kmp(int npz, double p_ref, const double *ak, const double *bk, int *kmp);
 
+/* ---- on-disk formats ----------------------------------------------------------
+ * Files are netCDF classic with 64-bit offsets (CDF-2: the NF90_64BIT_OFFSET
+ * choice of io/module_write_netcdf.F90:35 and FMS's restart default), written
+ * without libnetcdf.  Field arrays are device arrays with the halo'd layout
+ * (isd:ied, jsd:jed, nslab) per tile, tiles outermost; files hold the compute
+ * domain.  stream NULL: the context's stream.
+ *
+ * fv3_field_checksum: FMS mpp_chksum of each tile's compute domain -- the sum
+ * modulo 2^64 of the values' bit patterns read as two's-complement integers
+ * (float values: their 32-bit patterns, sign-extended).  out: ntiles values. */
+int fv3_field_checksum(fv3_ctx *ctx, const void *field, int nslab, int elem_bytes,
+                       unsigned long long *out, void *stream);
+
+/* The dycore's tracer restart, one file per tile named <prefix>.tile<t>.nc
+ * for t = tile_base .. tile_base + ntiles - 1 (upstream fv_io writes
+ * fv_tracer.res.tile<t>.nc beside the physics restarts of
+ * io/fv3atm_restart_io.F90:126-146): dimensions xaxis_1, yaxis_1, zaxis_1 and
+ * the record dimension Time; one double variable (Time, zaxis_1, yaxis_1,
+ * xaxis_1) per tracer named names[iq], with long_name, units and the FMS
+ * checksum attribute (16 upper-case hex digits, right-justified).
+ * q: (isd:ied, jsd:jed, npz, nq) per tile, float or double.  The read fills
+ * the compute domain of q (halos untouched) from the last record and, unless
+ * ignore_cksum (atmos_model.F90:193-194), fails when a tracer's checksum
+ * differs from its attribute. */
+int fv3_tracer_restart_write(fv3_ctx *ctx, const char *prefix, int tile_base, int nq,
+                             const char *const *names, const void *q, int elem_bytes, void *stream);
+int fv3_tracer_restart_read(fv3_ctx *ctx, const char *prefix, int tile_base, int nq,
+                            const char *const *names, void *q, int elem_bytes, int ignore_cksum,
+                            void *stream);
+
+/* The dycore history bundle in the layout of the write component's netCDF
+ * writer (io/module_write_netcdf.F90:50-560, cubed-sphere output grid):
+ * dimensions grid_xt, grid_yt, nchars, pfull, phalf, tile, time (fixed, 1);
+ * variables pfull, phalf (float), time, grid_xt, grid_yt (1..n), tile,
+ * time_iso, lon, lat (degrees, host (is:ie, js:je) per tile; NULL: zeros),
+ * then each field as (time, tile, [pfull,] grid_yt, grid_xt) of out_bytes
+ * (4: NF90_FLOAT, 8: NF90_DOUBLE) with long_name, units, coordinates and
+ * grid_mapping.  quantize_nsd > 0 applies BitRound to the 3-D fields
+ * (module_write_netcdf.F90:417-443, quantize_mode 'quantize_bitround': keep
+ * nsd mantissa bits, a half rounds away from zero) and records
+ * _QuantizeBitRoundNumberOfSignificantBits.  Global attributes: grid, im,
+ * jm and, when given, ak and bk (npz + 1 each). */
+typedef struct {
+  const char *name, *long_name, *units;
+  const void *data; /* device, (isd:ied, jsd:jed, nlev) per tile */
+  int nlev;         /* 1 (a 2-D field) or npz */
+  int elem_bytes;   /* 4 or 8 */
+} fv3_history_field;
+typedef struct {
+  int nfields;
+  const fv3_history_field *fields;
+  int out_bytes, quantize_nsd, tile_base;
+  const float *pfull, *phalf; /* host, npz and npz + 1 */
+  const double *lon, *lat, *ak, *bk;
+  double time;
+  const char *time_units, *time_iso;
+} fv3_history_args;
+int fv3_history_write(fv3_ctx *ctx, const char *path, const fv3_history_args *args, void *stream);
+
 /* Tangent-linear and adjoint of the FP64 advection + tracer remap step: the
  * derivatives of tracer_2d (per_level = 0; q_split > 0 forces nsplit) or
  * tracer_2d_1L (per_level = 1) followed by the tracer part of
