@@ -331,6 +331,31 @@ def field_checksum(ctx: Context, field, nslab, stream=None):
     return [int(v) for v in out]
 
 
+def dyn_core_fluxes(ctx: Context, form, n_split, bdt, hord_dp, uc, vc, delp, dp1, cx, cy, mfx, mfy,
+                    stream=None):
+    """dyn_core's accumulation of the tracer inputs over n_split acoustic
+    sub-steps of length bdt (see fv3_dyn_core_fluxes).  Device tensors of
+    the form's element type: uc, vc the C-grid winds of every sub-step;
+    delp in/out; dp1, cx, cy, mfx, mfy out."""
+    import torch
+    b = ctx.bd
+    nid, njd = b.ied - b.isd + 1, b.jed - b.jsd + 1
+    ni, nj = b.ie - b.is_ + 1, b.je - b.js + 1
+    per = ctx.ntiles * ctx.npz
+    dt = torch.float64 if form == FORM_PLAIN64 else torch.float32
+    need = dict(uc=n_split * per * (ni + 1) * njd, vc=n_split * per * nid * (nj + 1), delp=per * nid * njd,
+                dp1=per * nid * njd, cx=per * (ni + 1) * njd, cy=per * nid * (nj + 1), mfx=per * (ni + 1) * nj,
+                mfy=per * ni * (nj + 1))
+    ts = dict(uc=uc, vc=vc, delp=delp, dp1=dp1, cx=cx, cy=cy, mfx=mfx, mfy=mfy)
+    for k, t in ts.items():
+        _dev_field(k, t, need[k])
+        if t.dtype != dt:
+            raise TypeError(f"{k}: {dt} required for form {form}, got {t.dtype}")
+    check(_ffi.lib().fv3_dyn_core_fluxes(ctx.handle, form, n_split, bdt, hord_dp, ptr(uc), ptr(vc), ptr(delp),
+                                         ptr(dp1), ptr(cx), ptr(cy), ptr(mfx), ptr(mfy), _stream(stream)),
+          "fv3_dyn_core_fluxes")
+
+
 def _names(names):
     arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
     return arr

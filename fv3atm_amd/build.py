@@ -37,6 +37,7 @@ HIP_SOURCES = {
     "halo.hip": ["-ffp-contract=off"],
     "l2e.hip": ["-ffp-contract=off"],
     "io.hip": ["-ffp-contract=off"],
+    "dyncore.hip": ["-ffp-contract=off"],
 }
 
 

@@ -372,3 +372,26 @@ def l2e_state(case: Case, seed: int = 0, hydrostatic: bool = False) -> dict:
     if hydrostatic:
         del st["w"], st["ws"], st["delz"]
     return st
+
+
+def c_grid_winds(case, n_split: int, amp: float = 0.2):
+    """C-grid winds of n_split acoustic sub-steps whose Courant numbers over
+    the step add up to about the case's cx, cy: the wind the case's Courant
+    numbers imply (uc = cx dxa(upwind) / dt) modulated per sub-step and
+    level by 1 + amp sin(...).  Returns uc (n+1, nd, npz, ntiles, n_split),
+    vc (nd, n+1, npz, ntiles, n_split), Fortran order."""
+    n, nd, npz, nt = case.n, case.n + 2 * NG, case.npz, case.ntiles
+    cx = case.cx.reshape(n + 1, nd, npz, nt, order="F")
+    cy = case.cy.reshape(nd, n + 1, npz, nt, order="F")
+    uc = np.zeros((n + 1, nd, npz, nt, n_split), order="F")
+    vc = np.zeros((nd, n + 1, npz, nt, n_split), order="F")
+    for t, g in enumerate(case.grids):
+        dxa_m, dxa_p = g.dxa[NG - 1:NG + n, :], g.dxa[NG:NG + n + 1, :]       # cells i-1, i of face i
+        dya_m, dya_p = g.dya[:, NG - 1:NG + n], g.dya[:, NG:NG + n + 1]
+        u0 = np.where(cx[..., t] > 0, cx[..., t] * dxa_m[..., None], cx[..., t] * dxa_p[..., None]) / case.dt
+        v0 = np.where(cy[..., t] > 0, cy[..., t] * dya_m[..., None], cy[..., t] * dya_p[..., None]) / case.dt
+        for it in range(n_split):
+            ph = 2.0 * np.pi * (it + 0.5) / n_split + 0.37 * np.arange(npz)
+            uc[..., t, it] = u0 * (1.0 + amp * np.sin(ph))
+            vc[..., t, it] = v0 * (1.0 + amp * np.cos(ph))
+    return uc, vc

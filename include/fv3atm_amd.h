@@ -301,6 +301,31 @@ int fv3_lagrangian_to_eulerian_dev(fv3_ctx *ctx, int form, const fv3_l2e_args *a
  * level whose full-level pressure at p_ref exceeds 10 hPa (host arrays). */
 int fv3_kmp(int npz, double p_ref, const double *ak, const double *bk, int *kmp);
 
+/* The tracer path's inputs as dyn_core accumulates them over its n_split
+ * acoustic sub-steps (fv_dynamics hands them to tracer_2d_1L; dyn_core and
+ * d_sw are in the absent atmos_cubed_sphere submodule, reached from
+ * atmos_model.F90:879).  Each sub-step of length bdt turns the sub-step's
+ * C-grid winds into Courant numbers crx = bdt uc / dxa(upwind) and area
+ * fluxes xfx = crx dxa dy sin_sg(upwind) (cry, yfx likewise), transports
+ * delp with fv_tp_2d (order hord_dp, fluxes = area flux x delp face value,
+ * copy_corners acting on delp), adds crx, cry and the delp fluxes into cx,
+ * cy, mfx, mfy, updates delp by the flux divergence and, on six faces,
+ * refreshes delp's edge halos before the next sub-step (one tile: frozen).
+ * Device arrays, tiles outermost; element type double (FV3_FORM_PLAIN64) or
+ * float (FV3_FORM_FUSED32):
+ *   uc: n_split sub-steps of (is:ie+1, jsd:jed, npz) per tile (sub-step
+ *       outermost), vc: of (isd:ied, js:je+1, npz);
+ *   delp (isd:ied, jsd:jed, npz): in, the thickness at the start with its
+ *       halo filled; out, the thickness after the sub-steps;
+ *   dp1: out, delp at the start; cx (is:ie+1, jsd:jed, npz), cy (isd:ied,
+ *       js:je+1, npz), mfx (is:ie+1, js:je, npz), mfy (is:ie, js:je+1, npz):
+ *       out, the sums over the sub-steps.
+ * dp1 + (mfx(i) - mfx(i+1) + mfy(j) - mfy(j+1)) rarea is the final delp up
+ * to rounding: the consistency that keeps tracer_2d's constants constant. */
+int fv3_dyn_core_fluxes(fv3_ctx *ctx, int form, int n_split, double bdt, int hord_dp, const void *uc,
+                        const void *vc, void *delp, void *dp1, void *cx, void *cy, void *mfx, void *mfy,
+                        void *stream);
+
 /* ---- on-disk formats ----------------------------------------------------------
  * Files are netCDF classic with 64-bit offsets (CDF-2: the NF90_64BIT_OFFSET
  * choice of io/module_write_netcdf.F90:35 and FMS's restart default), written

@@ -100,6 +100,16 @@ int fv3o_fv_tp_2d(double *q, const double *crx, const double *cry, int npx, int 
  * NULL keeps halos as given at entry (single tile, frozen-halo policy). */
 typedef void (*fv3o_halo_fn)(void *ctx, double *field, int nslab, int k);
 
+/* ---- dyn_core's tracer inputs (fv3_oracle_dyn.c) ------------------------
+ * uc: n_split sub-steps of (is:ie+1, jsd:jed, npz) per tile, vc: of
+ * (isd:ied, js:je+1, npz); delp in: start thickness (halo filled), out: the
+ * thickness after the sub-steps; dp1 out: delp at entry; cx, cy, mfx, mfy
+ * out: the accumulated Courant numbers and delp fluxes. */
+int fv3o_dyn_core_fluxes(int n_split, double bdt, int hord_dp, const double *uc, const double *vc,
+                         double *delp, double *dp1, double *cx, double *cy, double *mfx, double *mfy,
+                         const fv3o_grid *g, const fv3o_bounds *bd, int ntiles, int npx, int npy,
+                         int npz, double lim_fac, fv3o_halo_fn halo, void *halo_ctx);
+
 /* Per-level sub-cycling (tracer_2d_1L).  ntiles tiles are stacked outermost
  * in every array (tile stride = the single-tile array size).  nsplit_out
  * (npz) receives nsplit(k) when non-NULL. */

@@ -67,6 +67,8 @@ def lib():
         L.fv3o_fillz.argtypes = [I, I, I, P, P]
         L.fv3o_lagrangian_to_eulerian_tracers.argtypes = [I, I, P, P, P, D, P, P, P, P, I, D, I]
         L.fv3o_lagrangian_to_eulerian_full.argtypes = [ctypes.POINTER(L2E)]
+        L.fv3o_dyn_core_fluxes.argtypes = [I, D, I, P, P, P, P, P, P, P, P, ctypes.POINTER(GridC),
+                                            ctypes.POINTER(Bounds), I, I, I, I, D, HALO_FN, P]
         L.fv3o_set_threads.argtypes = [I]
         L.fv3o_set_threads.restype = I
         _lib = L
@@ -175,6 +177,41 @@ def tracer_2d(case, variant="1L", q=None, dp1=None, cx=None, cy=None, mfx=None, 
         raise ValueError("oracle tracer_2d failed")
     arr["nsplit"] = nsplit
     return arr
+
+
+def dyn_core_fluxes(case, uc, vc, n_split, bdt, hord_dp=None, delp=None):
+    """The oracle's dyn_core accumulation (fv3o_dyn_core_fluxes) on copies.
+    uc: (n+1, nd, npz, ntiles, n_split), vc: (nd, n+1, npz, ntiles, n_split).
+    Returns dict(delp, dp1, cx, cy, mfx, mfy) with the case's array shapes."""
+    c = case
+    nd, nt = c.n + 2 * NG, c.ntiles
+    shp = lambda *s: s if nt > 1 else s[:-1]
+    z = lambda *s: np.zeros(shp(*s), dtype=np.float64, order="F")
+    out = dict(delp=np.array(c.dp1 if delp is None else delp, order="F", copy=True),
+               dp1=z(nd, nd, c.npz, nt), cx=z(c.n + 1, nd, c.npz, nt), cy=z(nd, c.n + 1, c.npz, nt),
+               mfx=z(c.n + 1, c.n, c.npz, nt), mfy=z(c.n, c.n + 1, c.npz, nt))
+    u = np.asfortranarray(uc, dtype=np.float64)
+    v = np.asfortranarray(vc, dtype=np.float64)
+    grids = (GridC * nt)(*[grid_struct(g) for g in c.grids])
+    bd = bounds(c.n)
+    halo_cb = HALO_FN()
+    keep = None
+    if nt == 6:
+        sh = SphereHalo(c.n)
+
+        def _halo(ctx, p, nslab, k):
+            buf = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_double)), shape=(6 * nslab * sh.s2,))
+            sh.fill(buf, nslab, nslab, 0)
+
+        keep = HALO_FN(_halo)
+        halo_cb = keep
+    rc = lib().fv3o_dyn_core_fluxes(n_split, bdt, c.hord if hord_dp is None else hord_dp, _p(u), _p(v),
+                                    _p(out["delp"]), _p(out["dp1"]), _p(out["cx"]), _p(out["cy"]),
+                                    _p(out["mfx"]), _p(out["mfy"]), grids, ctypes.byref(bd), nt, c.n + 1,
+                                    c.n + 1, c.npz, c.lim_fac, halo_cb, None)
+    if rc:
+        raise ValueError("oracle dyn_core_fluxes failed")
+    return out
 
 
 def remap_tracers(case, q, delp, pt=None, kord_tr=None, kord_tm=None, t_min=184.0, fill=False):

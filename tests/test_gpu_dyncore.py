@@ -1,0 +1,104 @@
+"""dyn_core's accumulation of the tracer inputs (fv3_dyn_core_fluxes) on the
+GPU against the FP64 oracle (fv3o_dyn_core_fluxes): bit for bit in FP64 on
+one tile and on six faces (delp halos refreshed between sub-steps), within
+FP32 rounding in FP32; then end to end, the outputs drive tracer_2d_1L and
+a constant tracer stays constant."""
+import numpy as np
+import pytest
+import torch
+
+import fv3atm_amd as fa
+from fv3atm_amd import cases
+from oracle import oracle
+from tests._util import rel_l2
+
+pytestmark = pytest.mark.gpu
+NG = 3
+KEYS = ("delp", "dp1", "cx", "cy", "mfx", "mfy")
+
+
+def _dev(a, dt):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=dt).ravel(order="F"))).cuda()
+
+
+def _run(case, form, n_split, uc, vc, bdt):
+    dt = np.float64 if form == fa.FORM_PLAIN64 else np.float32
+    ctx = fa.Context(case.grids, case.npz)
+    ref = oracle.dyn_core_fluxes(case, uc, vc, n_split, bdt)
+    dev = dict(uc=_dev(uc, dt), vc=_dev(vc, dt), delp=_dev(case.dp1, dt))
+    for k in KEYS[1:]:
+        dev[k] = torch.zeros(ref[k].size, dtype=dev["uc"].dtype, device="cuda")
+    fa.dyn_core_fluxes(ctx, form, n_split, bdt, case.hord, dev["uc"], dev["vc"], dev["delp"], dev["dp1"],
+                       dev["cx"], dev["cy"], dev["mfx"], dev["mfy"])
+    got = {k: dev[k].cpu().numpy().astype(np.float64).reshape(ref[k].shape, order="F") for k in KEYS}
+    ctx.close()
+    return got, ref
+
+
+def _interior(k, a):
+    # delp's corner halo blocks take copy_corners values and its halo the
+    # last exchange; compare the compute domain there, every face array whole
+    return a[NG:-NG, NG:-NG] if k in ("delp", "dp1") else a
+
+
+@pytest.mark.parametrize("nt,n_split", [(1, 6), (6, 3)])
+def test_fp64_matches_oracle(gpu_ready, nt, n_split):
+    case = cases.make_case(f"{nt}xC16L6", n=16, npz=6, nq=1, ntiles=nt, courant=1.8, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=8)
+    print(f"[dyn_core] FP64 {nt} tile(s) n_split={n_split}", flush=True)
+    uc, vc = cases.c_grid_winds(case, n_split)
+    got, ref = _run(case, fa.FORM_PLAIN64, n_split, uc, vc, case.dt / n_split)
+    for k in KEYS:
+        a, b = _interior(k, got[k]), _interior(k, ref[k])
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        print(f"  {k}: max |d| / max |ref| = {err:.2e}")
+        assert err <= 1e-12, k
+
+
+def test_fp32_error(gpu_ready):
+    case = cases.make_case("C32L8", n=32, npz=8, nq=1, courant=1.5, delp_kind=1, delp_amp=(0.05, 0.2), seed=2)
+    n_split = 4
+    uc, vc = cases.c_grid_winds(case, n_split)
+    got, ref = _run(case, fa.FORM_FUSED32, n_split, uc, vc, case.dt / n_split)
+    for k in KEYS:
+        e = rel_l2(_interior(k, got[k]), _interior(k, ref[k]))
+        print(f"  FP32 {k}: rel L2 {e:.2e}")
+        assert e <= 1e-6, k
+
+
+def test_constant_tracer_through_the_path(gpu_ready):
+    # the GPU's own outputs drive the GPU tracer_2d_1L: q = 1 stays 1 and the
+    # transport's thickness matches the accumulated delp
+    case = cases.make_case("C24L5", n=24, npz=5, nq=2, courant=2.2, delp_kind=1, delp_amp=(0.05, 0.2), seed=6)
+    n_split = 5
+    uc, vc = cases.c_grid_winds(case, n_split)
+    got, _ = _run(case, fa.FORM_PLAIN64, n_split, uc, vc, case.dt / n_split)
+    ctx = fa.Context(case.grids, case.npz)
+    a = {k: np.array(got[k], order="F", copy=True) for k in ("dp1", "cx", "cy", "mfx", "mfy")}
+    q = np.ones_like(case.q)
+    ns = fa.tracer_2d_1L(ctx, q, a["dp1"], a["mfx"], a["mfy"], a["cx"], a["cy"], case.n + 1, case.n + 1,
+                         case.npz, case.nq, case.hord)
+    ctx.close()
+    assert ns.max() >= 2
+    assert np.abs(q[NG:-NG, NG:-NG] - 1.0).max() <= 1e-13
+
+
+def test_argument_checks(gpu_ready):
+    case = cases.make_case("C8L2", n=8, npz=2, nq=1, seed=1)
+    ctx = fa.Context(case.grids, case.npz)
+    t = lambda n: torch.zeros(n, dtype=torch.float64, device="cuda")
+    big = t(4 * 2 * 15 * 15 * 4)
+    lib = fa._ffi.lib()
+    p = fa._ffi.ptr
+    for args, msg in (((0, 1.0, 8, big, big, t(400), big, big, big, big, big), b"n_split"),
+                      ((2, 1.0, 5, big, big, t(400), big, big, big, big, big), b"hord_dp"),
+                      ((2, -1.0, 8, big, big, t(400), big, big, big, big, big), b"bdt")):
+        n_split, bdt, hord, uc, vc, delp, dp1, cx, cy, mfx, mfy = args
+        rc = lib.fv3_dyn_core_fluxes(ctx.handle, fa.FORM_PLAIN64, n_split, bdt, hord, p(uc), p(vc), p(delp),
+                                     p(dp1), p(cx), p(cy), p(mfx), p(mfy), None)
+        assert rc != 0 and msg in lib.fv3_last_error()
+    d = t(400)
+    rc = lib.fv3_dyn_core_fluxes(ctx.handle, fa.FORM_PLAIN64, 2, 1.0, 8, p(big), p(big), p(d), p(d), p(big),
+                                 p(big), p(big), p(big), None)
+    assert rc != 0 and b"distinct" in lib.fv3_last_error()
+    ctx.close()
