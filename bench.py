@@ -351,6 +351,71 @@ def run_mixed_precision(args, torch, fa, cases):
         flush=True)
 
 
+def run_next_rows(args, torch, fa, case, d):
+    """The rows either side of the path at the workload's size, timed with
+    events on the context stream: dyn_core's accumulation of the transport
+    inputs (two acoustic sub-steps, FP32, winds derived from the workload's
+    Courant numbers) and, with --io, the tracer restart and history files."""
+    N, L = case.n, case.npz
+    nd = N + 6
+    out = {}
+    ctx = fa.Context(case.grids, L)
+    g = case.grids[0]
+    dxa = torch.from_numpy(np.ascontiguousarray(g.dxa.T)).cuda().float()
+    dya = torch.from_numpy(np.ascontiguousarray(g.dya.T)).cuda().float()
+    cx = d["cx"].view(L, nd, N + 1)
+    cy = d["cy"].view(L, N + 1, nd)
+    u0 = torch.where(cx > 0, cx * dxa[:, 2:N + 3], cx * dxa[:, 3:N + 4]) / case.dt
+    v0 = torch.where(cy > 0, cy * dya[2:N + 3, :], cy * dya[3:N + 4, :]) / case.dt
+    n_split = 2
+    uc = torch.stack([u0, 1.1 * u0]).contiguous()
+    vc = torch.stack([v0, 0.9 * v0]).contiguous()
+    del u0, v0
+    delp = d["dp1"].float()
+    dev = {k: torch.empty(n, dtype=torch.float32, device="cuda")
+           for k, n in (("dp1", delp.numel()), ("cx", cx.numel()), ("cy", cy.numel()),
+                        ("mfx", L * N * (N + 1)), ("mfy", L * N * (N + 1)))}
+    bdt = case.dt / n_split
+    d0 = delp.clone()
+    run = lambda: fa.dyn_core_fluxes(ctx, fa.FORM_FUSED32, n_split, bdt, case.hord, uc, vc, delp, dev["dp1"],
+                                     dev["cx"], dev["cy"], dev["mfx"], dev["mfy"])
+    run()
+    reps = 3
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        delp.copy_(d0)
+        run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps / n_split
+    # per cell and sub-step: uc, vc read; delp, cx, cy, mfx, mfy read and written
+    b = N * N * L * 4 * (2 + 2 * 5)
+    out["dyn_core"] = dict(ms_per_substep=round(ms, 3), achieved_GBps=round(b / ms / 1e6, 1),
+                           frac_hbm=round(b / (ms * 1e-3) / HBM_PEAK, 4),
+                           workload=f"C{N} L{L} FP32, {n_split} sub-steps, staged stage kernels")
+    log(f"dyn_core fluxes: {ms:.2f} ms per acoustic sub-step")
+    del uc, vc, dev, d0
+    if args.io:
+        import tempfile
+        with tempfile.TemporaryDirectory(dir="/tmp") as td:
+            q = d["q_in"][: 2 * L * nd * nd]
+            t0 = time.perf_counter()
+            fa.tracer_restart_write(ctx, td + "/fv_tracer.res", ["sphum", "liq_wat"], q)
+            tw = time.perf_counter() - t0
+            back = torch.empty_like(q)
+            t0 = time.perf_counter()
+            fa.tracer_restart_read(ctx, td + "/fv_tracer.res", ["sphum", "liq_wat"], back)
+            tr = time.perf_counter() - t0
+            fb = 2 * N * N * L * 8
+            out["restart_io"] = dict(tracers=2, file_bytes=fb, write_s=round(tw, 3), read_s=round(tr, 3),
+                                     write_GBps=round(fb / tw / 1e9, 2), read_GBps=round(fb / tr / 1e9, 2))
+            log(f"restart write {tw:.2f} s, read {tr:.2f} s for {fb / 1e9:.2f} GB")
+            del back
+    ctx.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,6 +445,8 @@ def main():
     ap.add_argument("--pipeline", type=int, default=None,
                     help="tracers per transport chunk, each chunk remapped beside the next chunk's "
                          "transport (-1: one round of the launch shape; 0: off, the library's default)")
+    ap.add_argument("--no-next", action="store_true", help="skip the dyn_core input timing")
+    ap.add_argument("--io", action="store_true", help="also time the tracer restart write and read")
     ap.add_argument("--mixed", action="store_true",
                     help="the C384 L64 Q10 FP32-against-FP64 error report instead of the C768 tile")
     args = ap.parse_args()
@@ -560,6 +627,9 @@ def main():
         out_line["speedup_vs_plain64"] = round(out["plain64_ms_per_step"] / ms, 3)
         out_line["speedup_vs_fused64"] = round(out["fused64_ms_per_step"] / ms, 3)
     ctx.close()
+    torch.cuda.empty_cache()
+    if not args.no_next:
+        out_line.update(run_next_rows(args, torch, fa, case, d))
     del d
     torch.cuda.empty_cache()
     if not args.no_cpu:
