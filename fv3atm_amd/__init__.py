@@ -356,6 +356,21 @@ def dyn_core_fluxes(ctx: Context, form, n_split, bdt, hord_dp, uc, vc, delp, dp1
           "fv3_dyn_core_fluxes")
 
 
+def global_mass(ctx: Context, q, nq, delp, stream=None):
+    """Per-tracer global mass sum(q * delp * area) over every tile, level and
+    compute-domain cell, FP64 with Neumaier compensation in a fixed order."""
+    import torch
+    per = ctx.ntiles * ctx.npz * _slab(ctx)
+    _dev_field("q", q, per * nq)
+    _dev_field("delp", delp, per)
+    if delp.dtype != torch.float64:
+        raise TypeError("delp: float64 required")
+    out = np.zeros(nq, dtype=np.float64)
+    check(_ffi.lib().fv3_global_mass(ctx.handle, ptr(q), nq, ptr(delp), q.element_size(),
+                                     out.ctypes.data_as(_ffi.DP), _stream(stream)), "fv3_global_mass")
+    return out
+
+
 def _names(names):
     arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
     return arr

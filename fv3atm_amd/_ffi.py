@@ -96,6 +96,7 @@ SIGNATURES = {
     "fv3_lagrangian_to_eulerian_dev": (I, [P, I, ctypes.POINTER(L2EArgs), P]),
     "fv3_kmp": (I, [I, D, P, P, IP]),
     "fv3_dyn_core_fluxes": (I, [P, I, I, D, I, P, P, P, P, P, P, P, P, P]),
+    "fv3_global_mass": (I, [P, P, I, P, I, DP, P]),
     "fv3_field_checksum": (I, [P, P, I, I, ctypes.POINTER(ctypes.c_ulonglong), P]),
     "fv3_tracer_restart_write": (I, [P, ctypes.c_char_p, I, I, ctypes.POINTER(ctypes.c_char_p), P, I, P]),
     "fv3_tracer_restart_read": (I, [P, ctypes.c_char_p, I, I, ctypes.POINTER(ctypes.c_char_p), P, I, I, P]),

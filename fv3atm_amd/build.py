@@ -38,6 +38,7 @@ HIP_SOURCES = {
     "l2e.hip": ["-ffp-contract=off"],
     "io.hip": ["-ffp-contract=off"],
     "dyncore.hip": ["-ffp-contract=off"],
+    "gsum.hip": ["-ffp-contract=off"],
 }
 
 

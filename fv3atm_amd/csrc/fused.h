@@ -18,7 +18,7 @@ int fused32_round(int cfg);
 
 // per-level sub-cycle counts on the device (cmax, nsplit; no scaling)
 template <typename T>
-int device_nsplit(Ctx &c, const T *cx, const T *cy, bool per_level, int q_split, T *cmax,
+int device_nsplit(Ctx &c, const T *cx, const T *cy, bool per_level, int q_split, double *cmax,
                   int *nsplit, T *frac);
 
 // Device scratch handed to a step by its caller instead of the context's

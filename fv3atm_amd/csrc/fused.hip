@@ -71,7 +71,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     off += (b + 255) & ~(size_t)255;
     return o;
   };
-  const size_t o_cmax = take(sizeof(T) * npz), o_frac = take(sizeof(T) * npz);
+  const size_t o_cmax = take(sizeof(double) * npz), o_frac = take(sizeof(T) * npz);
   const size_t o_ns = take(sizeof(int) * npz), o_flag = take(sizeof(int));
   const size_t head = off;
   const int nt = c.ntiles;
@@ -86,7 +86,8 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
                      "(fv3_step_dev allocates its own)", head + b_q + b_pt + b_col * nt, arena.bytes);
   char *base = ext ? arena.base : (char *)ctx_stage(c, head);
   if (!base) return 1;
-  T *cmax = (T *)(base + o_cmax), *frac = (T *)(base + o_frac);
+  double *cmax = (double *)(base + o_cmax);
+  T *frac = (T *)(base + o_frac);
   int *nspl = (int *)(base + o_ns), *flag = c.graph_safe ? c.step_flag : (int *)(base + o_flag);
   if (device_nsplit<T>(c, cx, cy, c.step_per_level, c.step_per_level ? 0 : c.step_q_split, cmax, nspl, frac))
     return 1;

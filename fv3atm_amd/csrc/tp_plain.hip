@@ -64,27 +64,30 @@ __device__ __forceinline__ void atomic_max_nonneg<float>(float *addr, float v) {
   atomicMax((unsigned int *)addr, (unsigned int)__float_as_int(v));
 }
 
-// per-level cmax: block reduction then one atomic per block (max is order free)
+// per-level cmax: block reduction then one atomic per block (max is order
+// free).  Evaluated in FP64 from the FP64 sin_sg for every form, so that
+// the FP32 form's sub-cycle counts differ from FP64's only through the
+// caller's own FP32 rounding of the Courant numbers.
 template <typename T>
-__global__ void __launch_bounds__(256) k_cmax(const T *cx, const T *cy, const T *sin_sg, Dims d,
-                                              int npz, long long gs2, T *cmax) {
+__global__ void __launch_bounds__(256) k_cmax(const T *cx, const T *cy, const double *sin_sg, Dims d,
+                                              int npz, long long gs2, double *cmax) {
   const int z = blockIdx.z, tile = z / npz, k = z % npz;
   const T *cxk = cx + ((long long)tile * npz + k) * d.sx;
   const T *cyk = cy + ((long long)tile * npz + k) * d.sy;
-  const T *s5 = sin_sg + (long long)tile * 9 * gs2 + 4 * gs2;
+  const double *s5 = sin_sg + (long long)tile * 9 * gs2 + 4 * gs2;
   const bool top = (k + 1) < npz / 6;
-  T m = T(0);
+  double m = 0.0;
   const int ni = d.ie - d.is + 1, nj = d.je - d.js + 1;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ni * nj; t += gridDim.x * blockDim.x) {
     const int i = d.is + t % ni, j = d.js + t / ni;
-    T ax = fabs(cxk[(long long)(j - d.jsd) * d.nix + (i - d.is)]);
-    T ay = fabs(cyk[(long long)(j - d.js) * d.nid + (i - d.isd)]);
+    const double ax = fabs((double)cxk[(long long)(j - d.jsd) * d.nix + (i - d.is)]);
+    const double ay = fabs((double)cyk[(long long)(j - d.js) * d.nid + (i - d.isd)]);
     if (top)
       m = mx3(m, ax, ay);
     else
-      m = mx2(m, mx2(ax, ay) + T(1) - s5[(long long)(j - d.jsd) * d.nid + (i - d.isd)]);
+      m = mx2(m, mx2(ax, ay) + 1.0 - s5[(long long)(j - d.jsd) * d.nid + (i - d.isd)]);
   }
-  __shared__ T red[256];
+  __shared__ double red[256];
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -96,22 +99,22 @@ __global__ void __launch_bounds__(256) k_cmax(const T *cx, const T *cy, const T 
 
 // nsplit(k) and 1/nsplit from cmax (tracer_2d_1L per level, tracer_2d global)
 template <typename T>
-__global__ void k_nsplit(const T *cmax, int npz, int per_level, int q_split, int *nsplit,
+__global__ void k_nsplit(const double *cmax, int npz, int per_level, int q_split, int *nsplit,
                          T *frac) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int glob = 1;
   if (!per_level) {
     if (q_split == 0) {
-      T c_global = cmax[0];
+      double c_global = cmax[0];
       if (npz != 1)
         for (int k = 1; k < npz; k++) c_global = mx2(cmax[k], c_global);
-      glob = (int)(T(1) + c_global);
+      glob = (int)(1.0 + c_global);
     } else {
       glob = q_split;
     }
   }
   for (int k = 0; k < npz; k++) {
-    int n = per_level ? (int)(T(1) + cmax[k]) : glob;
+    int n = per_level ? (int)(1.0 + cmax[k]) : glob;
     nsplit[k] = n;
     frac[k] = T(1) / T(n);
   }
@@ -124,16 +127,15 @@ __global__ void k_zero(T *p, int n) {
 }
 
 template <typename T>
-int device_nsplit(Ctx &c, const T *cx, const T *cy, bool per_level, int q_split, T *cmax,
+int device_nsplit(Ctx &c, const T *cx, const T *cy, bool per_level, int q_split, double *cmax,
                   int *nsplit, T *frac) {
   const Dims d = c.d;
   const int npz = c.npz, nt = c.ntiles;
-  hipLaunchKernelGGL(k_zero<T>, dim3(1), dim3(256), 0, c.stream, cmax, npz);
+  hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(256), 0, c.stream, cmax, npz);
   FV3_HIP(hipGetLastError());
   dim3 b(256), gr((unsigned)std::min(64, (d.ie - d.is + 1) * (d.je - d.js + 1) / 256 + 1), 1,
                   (unsigned)(npz * nt));
-  hipLaunchKernelGGL(k_cmax<T>, gr, b, 0, c.stream, cx, cy, grid_of<T>(c).sin_sg, d, npz,
-                     grid_of<T>(c).s2, cmax);
+  hipLaunchKernelGGL(k_cmax<T>, gr, b, 0, c.stream, cx, cy, c.grid64.sin_sg, d, npz, c.grid64.s2, cmax);
   FV3_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_nsplit<T>, dim3(1), dim3(1), 0, c.stream, cmax, npz, per_level ? 1 : 0,
                      q_split, nsplit, frac);
@@ -141,7 +143,7 @@ int device_nsplit(Ctx &c, const T *cx, const T *cy, bool per_level, int q_split,
   return 0;
 }
 template int device_nsplit<double>(Ctx &, const double *, const double *, bool, int, double *, int *, double *);
-template int device_nsplit<float>(Ctx &, const float *, const float *, bool, int, float *, int *, float *);
+template int device_nsplit<float>(Ctx &, const float *, const float *, bool, int, double *, int *, float *);
 
 template <typename T>
 static size_t align_up(size_t b) {
@@ -175,7 +177,7 @@ int tracer_2d_plain(Ctx &c, bool per_level, T *q, T *dp1, T *mfx, T *mfy, T *cx,
   const size_t o_rax = take(sizeof(T) * d.srx * npz * nt);
   const size_t o_ray = take(sizeof(T) * d.sry * npz * nt);
   const size_t o_dp2 = take(sizeof(T) * d.sdp * npz * nt);
-  const size_t o_cmax = take(sizeof(T) * npz);
+  const size_t o_cmax = take(sizeof(double) * npz);
   const size_t o_frac = take(sizeof(T) * npz);
   const size_t o_nspl = take(sizeof(int) * npz);
   const size_t o_lev = take(sizeof(int) * npz * nt);
@@ -193,7 +195,8 @@ int tracer_2d_plain(Ctx &c, bool per_level, T *q, T *dp1, T *mfx, T *mfy, T *cx,
   if (!base) return -1;
   T *xfx = (T *)(base + o_xfx), *yfx = (T *)(base + o_yfx);
   T *rax = (T *)(base + o_rax), *ray = (T *)(base + o_ray), *dp2 = (T *)(base + o_dp2);
-  T *cmax = (T *)(base + o_cmax), *fracd = (T *)(base + o_frac);
+  double *cmax = (double *)(base + o_cmax);
+  T *fracd = (T *)(base + o_frac);
   int *nspl = (int *)(base + o_nspl), *levs = (int *)(base + o_lev);
   T *qn2 = (T *)(base + o_qn2);
   Slab *pairs = (Slab *)(base + o_pairs);
@@ -236,11 +239,11 @@ int tracer_2d_plain(Ctx &c, bool per_level, T *q, T *dp1, T *mfx, T *mfy, T *cx,
       return -1;
   }
   // ---- cmax, nsplit, in-place scaling
-  FV3_HIP(hipMemsetAsync(cmax, 0, sizeof(T) * npz, s));
+  FV3_HIP(hipMemsetAsync(cmax, 0, sizeof(double) * npz, s));
   {
     dim3 b(256), gr((unsigned)std::min(64, (d.ie - d.is + 1) * (d.je - d.js + 1) / 256 + 1), 1,
                     (unsigned)(npz * nt));
-    hipLaunchKernelGGL(k_cmax<T>, gr, b, 0, s, cx, cy, g.sin_sg, d, npz, g.s2, cmax);
+    hipLaunchKernelGGL(k_cmax<T>, gr, b, 0, s, cx, cy, c.grid64.sin_sg, d, npz, c.grid64.s2, cmax);
     FV3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_nsplit<T>, dim3(1), dim3(1), 0, s, cmax, npz, per_level ? 1 : 0,
                        q_split, nspl, fracd);

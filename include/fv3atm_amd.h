@@ -326,6 +326,18 @@ int fv3_dyn_core_fluxes(fv3_ctx *ctx, int form, int n_split, double bdt, int hor
                         const void *vc, void *delp, void *dp1, void *cx, void *cy, void *mfx, void *mfy,
                         void *stream);
 
+/* Global tracer mass: out[iq] = the sum over every tile, level and
+ * compute-domain cell of q * delp * area (the product in that order, FP64),
+ * with Neumaier-compensated FP64 sums in a fixed reduction order -- the same
+ * result on every run, within a few units in the last place of the exact
+ * sum of the products.  It stands for the dycore's global mass sums (g_sum
+ * over mpp_global_sum; the reference imports its bit-reproducible mode,
+ * moving_nest/fv_moving_nest_utils.F90:35,41).  q (isd:ied, jsd:jed, npz, nq)
+ * and delp (isd:ied, jsd:jed, npz) per tile on the device, q float or
+ * double; out: host, nq values. */
+int fv3_global_mass(fv3_ctx *ctx, const void *q, int nq, const double *delp, int elem_bytes, double *out,
+                    void *stream);
+
 /* ---- on-disk formats ----------------------------------------------------------
  * Files are netCDF classic with 64-bit offsets (CDF-2: the NF90_64BIT_OFFSET
  * choice of io/module_write_netcdf.F90:35 and FMS's restart default), written
