@@ -267,6 +267,7 @@ int fv3_ctx_destroy(fv3_ctx *h) {
   (void)hipFree(c.gfac32);
   (void)hipFree(c.halo_dst);
   (void)hipFree(c.halo_src);
+  if (c.halo_adj) (void)hipFree(c.halo_adj);
   (void)hipFree(c.scratch);
   (void)hipFree(c.stage);
   (void)hipFree(c.step_flag);

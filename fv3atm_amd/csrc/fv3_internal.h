@@ -110,6 +110,10 @@ struct Ctx {
   // on first use
   void *halo_tab[4] = {nullptr, nullptr, nullptr, nullptr};
   int halo_tab_n[4] = {0, 0, 0, 0};
+  // the edge-halo table grouped by source cell (the adjoint's transposed
+  // refresh), built on first use: distinct sources, offsets, halo cells
+  void *halo_adj = nullptr;
+  int halo_adj_nsrc = 0;
   // the fused step without its remap (the host-array transport entry points)
   // and the sub-cycle counts of the last step that derived them on the host
   bool transport_only = false;

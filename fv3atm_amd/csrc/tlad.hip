@@ -315,16 +315,49 @@ struct AdjTemps {
   double *Fx, *Fy;         // averaged-flux adjoints (smx, smy)
   double *fx, *fx2, *fy, *fy2;  // sweep-output adjoints (smx, sx, smy, sy)
   double *qj, *qi;         // (srx, sry)
+  double *cg;              // 36 per slab: adjoints read through the corner blocks
 };
+
+// corner-block slot (0..35) of cell (i, j): block 0 sw, 1 se, 2 ne, 3 nw, then
+// 3 x 3 cells i fastest; -1 outside the corner blocks
+__device__ __forceinline__ int corner_slot(int i, int j, int npx, int npy) {
+  const bool ilo = i <= 0 && i >= 1 - NG, ihi = i >= npx && i <= npx + NG - 1;
+  const bool jlo = j <= 0 && j >= 1 - NG, jhi = j >= npy && j <= npy + NG - 1;
+  int blk;
+  if (ilo && jlo) blk = 0;
+  else if (ihi && jlo) blk = 1;
+  else if (ihi && jhi) blk = 2;
+  else if (ilo && jhi) blk = 3;
+  else return -1;
+  const int a = ilo ? i - (1 - NG) : i - npx, b = jlo ? j - (1 - NG) : j - npy;
+  return blk * 9 + b * 3 + a;
+}
 
 // Reverse of one sub-cycle on the active slabs of a chunk: qbar (interior of
 // each slab) is replaced by the adjoint of the sub-cycle input.
+// With full (six faces) the adjoint of every cell the sub-cycle read is kept
+// in qbar, halo cells included (the halo transposes it onto the neighbour
+// faces afterwards); an adjoint read through a corner-block cell goes to the
+// cell copy_corners took its value from.  Without it (one tile, frozen
+// halos) only the interior is written.
 int subcycle_adjoint(hipStream_t s, const Geo &G, const LevelData &L, const int *zs, int np,
                      double *qbar, const SweepW &W, const AdjTemps &A, const double *dpa,
-                     const double *dpb) {
+                     const double *dpb, bool full) {
   const Dims d = G.d;
   const int npz = G.npz;
   const double *area = G.area, *rarea = G.rarea;
+  const unsigned corners = d.bounded ? 0u : d.corners;
+  // corner-block adjoints of sweep direction dir go to their sources
+  auto redirect = [&](int dir) {
+    return box(s, [=] __device__(int slot, int, int p) {
+          const int blk = slot / 9, cc = slot % 9;
+          const int i = (blk == 0 || blk == 3 ? 1 - NG : d.npx) + cc % 3;
+          const int j = (blk == 0 || blk == 1 ? 1 - NG : d.npy) + cc / 3;
+          int si, sj;
+          if (corner_src(i, j, d.npx, d.npy, dir, corners, si, sj))
+            A.qa[(long long)p * d.s2 + (long long)(sj - d.jsd) * d.nid + (si - d.isd)] += A.cg[(long long)p * 36 + slot];
+        }, 0, 35, 0, 0, np);
+  };
   // update: qa = qbar dp1/dp2 on the interior, 0 elsewhere
   if (box(s, [=] __device__(int i, int j, int p) {
         const int z = zs[p], k = z % npz;
@@ -402,7 +435,8 @@ int subcycle_adjoint(hipStream_t s, const Geo &G, const LevelData &L, const int 
         A.fx2[(long long)p * d.sx + o] += L.xfx[(long long)k * d.sx + o] * (b(f) - b(f - 1));
       }, d.is, d.ie + 1, d.jsd, d.jed, np))
     return -1;
-  // inner x-sweep: qa(m, j) += sum of W_fx2(f, j)[m-f+3] fx2_bar(f, j) (interior m only)
+  // inner x-sweep: qa(m, j) += sum of W_fx2(f, j)[m-f+3] fx2_bar(f, j) (the
+  // interior; with full every cell, corner-block reads set aside in cg)
   if (box(s, [=] __device__(int m, int j, int p) {
         double acc = 0.0;
         for (int f = m - 2; f <= m + 3; f++) {
@@ -410,9 +444,15 @@ int subcycle_adjoint(hipStream_t s, const Geo &G, const LevelData &L, const int 
           const long long face = (long long)p * d.sx + (long long)(j - d.jsd) * d.nix + (f - d.is);
           acc += W.fx2[face * 6 + (m - f + 3)] * A.fx2[face];
         }
+        int si, sj;
+        if (full && corners && corner_src(m, j, d.npx, d.npy, 1, corners, si, sj)) {
+          A.cg[(long long)p * 36 + corner_slot(m, j, d.npx, d.npy)] = acc;
+          return;
+        }
         A.qa[(long long)p * d.s2 + (long long)(j - d.jsd) * d.nid + (m - d.isd)] += acc;
-      }, d.is, d.ie, d.js, d.je, np))
+      }, full ? d.isd : d.is, full ? d.ied : d.ie, full ? d.jsd : d.js, full ? d.jed : d.je, np))
     return -1;
+  if (full && corners && redirect(1)) return -1;
   // outer x-sweep: qi_bar(m, j) = sum of W_fx(f, j)[m-f+3] fx_bar(f, j)
   if (box(s, [=] __device__(int m, int j, int p) {
         double acc = 0.0;
@@ -445,7 +485,7 @@ int subcycle_adjoint(hipStream_t s, const Geo &G, const LevelData &L, const int 
       }, d.isd, d.ied, d.js, d.je + 1, np))
     return -1;
   // inner y-sweep: qa(i, m) += sum of W_fy2(i, f)[m-f+3] fy2_bar(i, f); then
-  // qbar <- qa on the interior
+  // qbar <- qa on the interior (with full: on every cell)
   if (box(s, [=] __device__(int i, int m, int p) {
         double acc = 0.0;
         for (int f = m - 2; f <= m + 3; f++) {
@@ -454,10 +494,47 @@ int subcycle_adjoint(hipStream_t s, const Geo &G, const LevelData &L, const int 
           acc += W.fy2[face * 6 + (m - f + 3)] * A.fy2[face];
         }
         const long long o = (long long)(m - d.jsd) * d.nid + (i - d.isd);
-        qbar[(long long)zs[p] * d.s2 + o] = A.qa[(long long)p * d.s2 + o] + acc;
-      }, d.is, d.ie, d.js, d.je, np))
+        if (!full) {
+          qbar[(long long)zs[p] * d.s2 + o] = A.qa[(long long)p * d.s2 + o] + acc;
+          return;
+        }
+        int si, sj;
+        if (corners && corner_src(i, m, d.npx, d.npy, 2, corners, si, sj)) {
+          A.cg[(long long)p * 36 + corner_slot(i, m, d.npx, d.npy)] = acc;
+          return;
+        }
+        A.qa[(long long)p * d.s2 + o] += acc;
+      }, full ? d.isd : d.is, full ? d.ied : d.ie, full ? d.jsd : d.js, full ? d.jed : d.je, np))
+    return -1;
+  if (!full) return 0;
+  if (corners && redirect(2)) return -1;
+  if (box(s, [=] __device__(int i, int j, int p) {
+        const long long o = (long long)(j - d.jsd) * d.nid + (i - d.isd);
+        qbar[(long long)zs[p] * d.s2 + o] = A.qa[(long long)p * d.s2 + o];
+      }, d.isd, d.ied, d.jsd, d.jed, np))
     return -1;
   return 0;
+}
+
+// Transposed edge-halo refresh of six stacked tiles: every interior cell adds
+// the adjoints of the halo cells that copied it (in the table's order, so the
+// result does not depend on scheduling), then every cell outside the
+// interior is cleared.  usrc: the distinct source cells; ptr/dst: the halo
+// cells of each (tile << 40 | cell, as the forward table).
+__global__ void __launch_bounds__(256) k_halo_adjoint(double *q, long long tile_stride, long long s2,
+                                                      const long long *usrc, const int *ptr, const long long *dst,
+                                                      int nsrc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nsrc) return;
+  const long long sl = blockIdx.y;
+  double acc = 0.0;
+  for (int k = ptr[e]; k < ptr[e + 1]; k++) {
+    const long long td = dst[k] >> 40, cd = dst[k] & ((1ll << 40) - 1);
+    acc += q[td * tile_stride + sl * s2 + cd];
+  }
+  const long long ts = usrc[e] >> 40, cs = usrc[e] & ((1ll << 40) - 1);
+  double &v = q[ts * tile_stride + sl * s2 + cs];
+  v = v + acc;
 }
 
 // ---------------------------------------------------------------- remap
@@ -884,12 +961,19 @@ static int tlad_nsplit(Ctx &c, bool per_level, int q_split, const double *cx, co
 }
 
 // Scaled winds, area fluxes and the thickness of every sub-cycle.
-static int tlad_prepare(Ctx &c, const double *cx, const double *cy, const double *mfx,
+static int tlad_prepare(Ctx &c, int t, const double *cx, const double *cy, const double *mfx,
                         const double *mfy, const double *dp1, char *&cursor, TlAdPlan &P) {
   const Dims d = c.d;
   const int npz = c.npz;
   hipStream_t s = c.stream;
   const GridDev<double> g = grid_of<double>(c);
+  // tile t's inputs and metrics
+  cx += (long long)t * d.sx * npz;
+  cy += (long long)t * d.sy * npz;
+  mfx += (long long)t * d.smx * npz;
+  mfy += (long long)t * d.smy * npz;
+  dp1 += (long long)t * d.s2 * npz;
+  const long long tm = (long long)t * g.s2;
   auto take = [&](size_t bytes) {
     char *r = cursor;
     cursor += (bytes + 255) & ~(size_t)255;
@@ -901,7 +985,8 @@ static int tlad_prepare(Ctx &c, const double *cx, const double *cy, const double
   double *mfxs = (double *)take(sizeof(double) * d.smx * npz), *mfys = (double *)take(sizeof(double) * d.smy * npz);
   const int *nspl = P.d_nsplit;
   const double *frac = P.d_frac;
-  const double *dxa = g.dxa, *dya = g.dya, *gdx = g.dx, *gdy = g.dy, *sg = g.sin_sg, *area = g.area;
+  const double *dxa = g.dxa + tm, *dya = g.dya + tm, *gdx = g.dx + (long long)t * g.sdx,
+               *gdy = g.dy + (long long)t * g.sdy, *sg = g.sin_sg + 9 * tm, *area = g.area + tm;
   const long long gs2 = g.s2;
   const int *nsp = nspl;
   const double *fr = frac;
@@ -963,7 +1048,7 @@ static int tlad_prepare(Ctx &c, const double *cx, const double *cy, const double
   // on levels with nsplit >= it (the running dp1 <- dp2 of tracer_2d)
   P.dp_hist = (double *)take(sizeof(double) * d.s2 * npz * (P.nmax + 1));
   FV3_HIP(hipMemcpyAsync(P.dp_hist, dp1, sizeof(double) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
-  const double *rarea = g.rarea;
+  const double *rarea = g.rarea + tm;
   for (int it = 1; it <= P.nmax; it++) {
     const double *prev = P.dp_hist + (long long)(it - 1) * npz * d.s2;
     double *cur = P.dp_hist + (long long)it * npz * d.s2;
@@ -995,12 +1080,65 @@ static size_t tlad_prep_bytes(const Dims &d, int npz, int nmax) {
 }
 
 static int tlad_check(Ctx &c, int hord, int kord_tr) {
-  if (c.ntiles != 1) return set_error("tangent-linear/adjoint: single-tile context only (frozen halo)");
+  if (c.ntiles != 1 && !(c.ntiles == 6 && c.halo_n > 0))
+    return set_error("tangent-linear/adjoint: a single-tile or six-face context is required");
   if (hord < 7 || hord > 13) return set_error("hord=%d: only the monotone orders 7..13 are implemented", hord);
   if (kord_tr != 0 && !(kord_tr > 7 && (kord_tr <= 10 || kord_tr > 16)))
     return set_error("kord_tr=%d: implemented orders are 8, 9, 10 and > 16 (0 skips the remap)", kord_tr);
   if (kord_tr != 0 && c.npz < 6) return set_error("remap tangent-linear/adjoint needs npz >= 6");
   return 0;
+}
+
+// The halo table grouped by source cell, for k_halo_adjoint (built once per
+// context from the forward table).
+static int halo_adjoint_table(Ctx &c, const long long **usrc, const int **ptr, const long long **dst, int *nsrc) {
+  if (!c.halo_adj) {
+    const int n = c.halo_n;
+    std::vector<long long> hd(n), hs(n);
+    FV3_HIP(hipMemcpy(hd.data(), c.halo_dst, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    FV3_HIP(hipMemcpy(hs.data(), c.halo_src, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    std::vector<int> ord(n);
+    for (int e = 0; e < n; e++) ord[e] = e;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return hs[x] < hs[y]; });
+    std::vector<long long> us, ds;
+    std::vector<int> pt;
+    for (int k = 0; k < n; k++) {
+      const int e = ord[k];
+      if (us.empty() || us.back() != hs[e]) {
+        us.push_back(hs[e]);
+        pt.push_back(k);
+      }
+      ds.push_back(hd[e]);
+    }
+    pt.push_back(n);
+    const int nu = (int)us.size();
+    const size_t b_us = ((sizeof(long long) * nu) + 255) & ~(size_t)255;
+    const size_t b_pt = ((sizeof(int) * (nu + 1)) + 255) & ~(size_t)255;
+    void *buf = nullptr;
+    FV3_HIP(hipMalloc(&buf, b_us + b_pt + sizeof(long long) * std::max(n, 1)));
+    char *cb = (char *)buf;
+    FV3_HIP(hipMemcpy(cb, us.data(), sizeof(long long) * nu, hipMemcpyHostToDevice));
+    FV3_HIP(hipMemcpy(cb + b_us, pt.data(), sizeof(int) * (nu + 1), hipMemcpyHostToDevice));
+    FV3_HIP(hipMemcpy(cb + b_us + b_pt, ds.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    c.halo_adj = buf;
+    c.halo_adj_nsrc = nu;
+  }
+  const int nu = c.halo_adj_nsrc;
+  const size_t b_us = ((sizeof(long long) * nu) + 255) & ~(size_t)255;
+  const size_t b_pt = ((sizeof(int) * (nu + 1)) + 255) & ~(size_t)255;
+  char *cb = (char *)c.halo_adj;
+  *usrc = (const long long *)cb;
+  *ptr = (const int *)(cb + b_us);
+  *dst = (const long long *)(cb + b_us + b_pt);
+  *nsrc = nu;
+  return 0;
+}
+
+// per-tile metric views of the context's FP64 grid
+static Geo geo_of(const Ctx &c, int t) {
+  const GridDev<double> &g = c.grid64;
+  const long long tm = (long long)t * g.s2;
+  return Geo{c.d, g.area + tm, g.rarea + tm, g.dxa + tm, g.dya + tm, c.npz};
 }
 
 int step_tl(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const double *dq0,
@@ -1009,12 +1147,12 @@ int step_tl(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const
             const double *bk, double ptop, int hord, int kord_tr) {
   if (tlad_check(c, hord, kord_tr)) return 1;
   const Dims d = c.d;
-  const int npz = c.npz;
+  const int npz = c.npz, nt = c.ntiles;
   hipStream_t s = c.stream;
-  const long long qsize = d.s2 * npz * nq;
-  TlAdPlan P;
-  if (tlad_nsplit(c, per_level, q_split, cx, cy, P)) return 1;
-  const int nb = P.nmax;
+  const long long qt = d.s2 * npz * nq, qsize = qt * nt;  // one tile, all tiles
+  TlAdPlan P0;
+  if (tlad_nsplit(c, per_level, q_split, cx, cy, P0)) return 1;
+  const int nb = P0.nmax;
   const int nslab = npz * nq;
   const size_t per_slab = sizeof(D1) * (d.sy + d.sry + d.smx + d.sx + d.srx + d.smy);
   const int chunk = (int)std::max<long long>(1, std::min<long long>(std::min(nslab, 65535), (2ll << 30) / (long long)per_slab));
@@ -1022,12 +1160,14 @@ int step_tl(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const
   const size_t remap_b = kord_tr ? sizeof(double) * ncol * (4 * (npz + 1)) + sizeof(D1) * ncol * (7 * (npz + 1)) +
                                        3 * ncol * (npz + 1) + 4096
                                  : 0;
-  const size_t bytes = tlad_prep_bytes(d, npz, nb) + 2 * sizeof(double) * qsize + sizeof(int) * nslab +
-                       per_slab * chunk + remap_b + 8192;
+  const size_t bytes = tlad_prep_bytes(d, npz, nb) * nt + 2 * sizeof(double) * qsize + sizeof(int) * nslab +
+                       per_slab * chunk + remap_b + 8192 * nt;
   char *base = (char *)ctx_scratch(c, bytes);
   if (!base) return 1;
   char *cur = base;
-  if (tlad_prepare(c, cx, cy, mfx, mfy, dp1, cur, P)) return 1;
+  std::vector<TlAdPlan> P(nt, P0);
+  for (int t = 0; t < nt; t++)
+    if (tlad_prepare(c, t, cx, cy, mfx, mfy, dp1, cur, P[t])) return 1;
   auto take = [&](size_t b) {
     char *r = cur;
     cur += (b + 255) & ~(size_t)255;
@@ -1042,43 +1182,46 @@ int step_tl(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const
         const long long o = (long long)z * d.s2 + (long long)(j - d.jsd) * d.nid + (i - d.isd);
         const bool in = i >= d.is && i <= d.ie && j >= d.js && j <= d.je;
         wt[o] = in ? dq0[o] : 0.0;
-      }, d.isd, d.ied, d.jsd, d.jed, nslab))
+      }, d.isd, d.ied, d.jsd, d.jed, nslab * nt))
     return 1;
-  const Geo G{d, grid_of<double>(c).area, grid_of<double>(c).rarea, grid_of<double>(c).dxa,
-              grid_of<double>(c).dya, npz};
   const int ord_in = hord == 10 ? 8 : hord, ord_ou = hord;
   std::vector<int> act;
-  for (int it = 1; it <= P.nmax; it++) {
+  for (int it = 1; it <= nb; it++) {
     act.clear();
     for (int iq = 0; iq < nq; iq++)
       for (int k = 0; k < npz; k++)
-        if (P.nsplit[k] >= it) act.push_back(iq * npz + k);
+        if (P0.nsplit[k] >= it) act.push_back(iq * npz + k);
     FV3_HIP(hipMemcpyAsync(zs, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, s));
-    const double *dpa = P.dp_hist + (long long)(it - 1) * npz * d.s2, *dpb = P.dp_hist + (long long)it * npz * d.s2;
-    for (size_t p0 = 0; p0 < act.size(); p0 += chunk) {
-      const int np = (int)std::min<size_t>(chunk, act.size() - p0);
-      SubTemps<D1> T{tmp, tmp + d.sy * chunk, tmp + (d.sy + d.sry) * chunk,
-                     tmp + (d.sy + d.sry + d.smx) * chunk, tmp + (d.sy + d.sry + d.smx + d.sx) * chunk,
-                     tmp + (d.sy + d.sry + d.smx + d.sx + d.srx) * chunk};
-      if (subcycle_forward<D1>(s, G, P.L, zs + p0, np, wv, wt, T, dpa, dpb, ord_in, ord_ou, true)) return 1;
+    // six faces: every sub-cycle starts from halos filled from the
+    // neighbouring faces' current interiors, values and tangents alike
+    if (nt > 1 && (halo_fill<double>(c, wv, qt, nullptr, 0, nslab) || halo_fill<double>(c, wt, qt, nullptr, 0, nslab)))
+      return 1;
+    for (int t = 0; t < nt; t++) {
+      const Geo G = geo_of(c, t);
+      const double *dpa = P[t].dp_hist + (long long)(it - 1) * npz * d.s2, *dpb = P[t].dp_hist + (long long)it * npz * d.s2;
+      for (size_t p0 = 0; p0 < act.size(); p0 += chunk) {
+        const int np = (int)std::min<size_t>(chunk, act.size() - p0);
+        SubTemps<D1> T{tmp, tmp + d.sy * chunk, tmp + (d.sy + d.sry) * chunk,
+                       tmp + (d.sy + d.sry + d.smx) * chunk, tmp + (d.sy + d.sry + d.smx + d.sx) * chunk,
+                       tmp + (d.sy + d.sry + d.smx + d.sx + d.srx) * chunk};
+        if (subcycle_forward<D1>(s, G, P[t].L, zs + p0, np, wv + t * qt, wt + t * qt, T, dpa, dpb, ord_in, ord_ou,
+                                 true))
+          return 1;
+      }
     }
     FV3_HIP(hipStreamSynchronize(s));  // act is reused by the next sub-cycle
   }
+  // outputs (halo cells keep the transported values; the remap rewrites the interior)
+  FV3_HIP(hipMemcpyAsync(q1, wv, sizeof(double) * qsize, hipMemcpyDeviceToDevice, s));
+  FV3_HIP(hipMemcpyAsync(dq1, wt, sizeof(double) * qsize, hipMemcpyDeviceToDevice, s));
+  if (delp_out) FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz * nt, hipMemcpyDeviceToDevice, s));
   if (!kord_tr) {
-    FV3_HIP(hipMemcpyAsync(q1, wv, sizeof(double) * qsize, hipMemcpyDeviceToDevice, s));
-    FV3_HIP(hipMemcpyAsync(dq1, wt, sizeof(double) * qsize, hipMemcpyDeviceToDevice, s));
-    if (delp_out) FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
     FV3_HIP(hipStreamSynchronize(s));
     return 0;
   }
-  // remap (outputs: halo cells keep the transported values)
-  FV3_HIP(hipMemcpyAsync(q1, wv, sizeof(double) * qsize, hipMemcpyDeviceToDevice, s));
-  FV3_HIP(hipMemcpyAsync(dq1, wt, sizeof(double) * qsize, hipMemcpyDeviceToDevice, s));
-  if (delp_out) FV3_HIP(hipMemcpyAsync(delp_out, dp1, sizeof(double) * d.s2 * npz, hipMemcpyDeviceToDevice, s));
   RemapTL r{};
   r.km = npz; r.nq = nq; r.ni = d.ie - d.is + 1; r.is = d.is; r.js = d.js; r.isd = d.isd; r.jsd = d.jsd;
   r.nid = d.nid; r.kord = kord_tr; r.ncol = ncol; r.s2 = d.s2; r.ak = ak; r.bk = bk; r.ptop = ptop;
-  r.dp1 = dp1; r.delp_out = delp_out; r.qv = wv; r.qt = wt; r.ov = q1; r.ot = dq1;
   r.pe1 = (double *)take(sizeof(double) * ncol * (npz + 1));
   r.pe2 = (double *)take(sizeof(double) * ncol * (npz + 1));
   r.d1 = (double *)take(sizeof(double) * ncol * (npz + 1));
@@ -1089,8 +1232,14 @@ int step_tl(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const
   r.dm = nullptr;
   r.flags = (uint8_t *)take(3 * ncol * npz);
   if ((size_t)(cur - base) > bytes) return set_error("tangent-linear scratch layout overflow");
-  hipLaunchKernelGGL(k_remap_tl, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, s, r);
-  FV3_HIP(hipGetLastError());
+  for (int t = 0; t < nt; t++) {  // column-local: one tile after another on the same scratch
+    const long long td = (long long)t * d.s2 * npz;
+    r.dp1 = dp1 + td;
+    r.delp_out = delp_out ? delp_out + td : nullptr;
+    r.qv = wv + t * qt; r.qt = wt + t * qt; r.ov = q1 + t * qt; r.ot = dq1 + t * qt;
+    hipLaunchKernelGGL(k_remap_tl, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, s, r);
+    FV3_HIP(hipGetLastError());
+  }
   FV3_HIP(hipStreamSynchronize(s));
   return 0;
 }
@@ -1101,39 +1250,44 @@ int step_ad(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const
             double ptop, int hord, int kord_tr) {
   if (tlad_check(c, hord, kord_tr)) return 1;
   const Dims d = c.d;
-  const int npz = c.npz;
+  const int npz = c.npz, nt = c.ntiles;
+  const bool faces = nt > 1;
   hipStream_t s = c.stream;
-  const long long qsize = d.s2 * npz * nq;
-  TlAdPlan P;
-  if (tlad_nsplit(c, per_level, q_split, cx, cy, P)) return 1;
-  const int nb = P.nmax;
+  const long long qt = d.s2 * npz * nq, qsize = qt * nt;
+  TlAdPlan P0;
+  if (tlad_nsplit(c, per_level, q_split, cx, cy, P0)) return 1;
+  const int nb = P0.nmax;
   const int nslab = npz * nq;
   const long long ncol = (long long)(d.ie - d.is + 1) * (d.je - d.js + 1);
-  // per slab: primal temps, 4 x 6 weights, adjoint temps
-  const long long faces = d.sy + d.smx + d.sx + d.smy;
+  // per slab: primal temps, 4 x 6 weights, adjoint temps, corner gathers
+  const long long faces_n = d.sy + d.smx + d.sx + d.smy;
   const size_t per_slab = sizeof(double) * (d.sy + d.sry + d.smx + d.sx + d.srx + d.smy) +
-                          sizeof(double) * 6 * faces +
-                          sizeof(double) * (d.s2 + d.smx + d.smy + d.smx + d.sx + d.smy + d.sy + d.srx + d.sry);
+                          sizeof(double) * 6 * faces_n +
+                          sizeof(double) * (d.s2 + d.smx + d.smy + d.smx + d.sx + d.smy + d.sy + d.srx + d.sry + 36);
   const int chunk = (int)std::max<long long>(1, std::min<long long>(std::min(nslab, 65535), (2ll << 30) / (long long)per_slab));
   const size_t remap_b = kord_tr ? sizeof(double) * ncol * (npz + 1) * 16 + 3 * ncol * (npz + 1) + 4096 : 0;
   // checkpoints: the state entering every sub-cycle, plus the transported state
-  const size_t bytes = tlad_prep_bytes(d, npz, nb) + sizeof(double) * qsize * (nb + 2) + sizeof(int) * nslab +
-                       per_slab * chunk + remap_b + 8192;
+  const size_t bytes = tlad_prep_bytes(d, npz, nb) * nt + sizeof(double) * qsize * (nb + 2) + sizeof(int) * nslab +
+                       per_slab * chunk + remap_b + 8192 * nt;
+  const long long *hu = nullptr, *hd = nullptr;
+  const int *hp = nullptr;
+  int hn = 0;
+  if (faces && halo_adjoint_table(c, &hu, &hp, &hd, &hn)) return 1;
   char *base = (char *)ctx_scratch(c, bytes);
   if (!base) return 1;
   char *cur = base;
-  if (tlad_prepare(c, cx, cy, mfx, mfy, dp1, cur, P)) return 1;
+  std::vector<TlAdPlan> P(nt, P0);
+  for (int t = 0; t < nt; t++)
+    if (tlad_prepare(c, t, cx, cy, mfx, mfy, dp1, cur, P[t])) return 1;
   auto take = [&](size_t b) {
     char *r = cur;
     cur += (b + 255) & ~(size_t)255;
     return r;
   };
-  double *ck = (double *)take(sizeof(double) * qsize * (P.nmax + 1));  // ck[it-1]: input of sub-cycle it; ck[nmax]: final
+  double *ck = (double *)take(sizeof(double) * qsize * (nb + 1));  // ck[it-1]: input of sub-cycle it; ck[nmax]: final
   double *qbar = (double *)take(sizeof(double) * qsize);
   int *zs = (int *)take(sizeof(int) * nslab);
   char *tmp = take(per_slab * chunk);
-  const Geo G{d, grid_of<double>(c).area, grid_of<double>(c).rarea, grid_of<double>(c).dxa,
-              grid_of<double>(c).dya, npz};
   const int ord_in = hord == 10 ? 8 : hord, ord_ou = hord;
   // temporaries
   double *tp = (double *)tmp;
@@ -1141,7 +1295,7 @@ int step_ad(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const
                      tp + (d.sy + d.sry + d.smx + d.sx) * chunk, tp + (d.sy + d.sry + d.smx + d.sx + d.srx) * chunk};
   double *wp = tp + (d.sy + d.sry + d.smx + d.sx + d.srx + d.smy) * chunk;
   SweepW W{wp, wp + 6 * d.sy * chunk, wp + 6 * (d.sy + d.smx) * chunk, wp + 6 * (d.sy + d.smx + d.sx) * chunk};
-  double *ap = wp + 6 * faces * chunk;
+  double *ap = wp + 6 * faces_n * chunk;
   AdjTemps A;
   A.qa = ap;
   A.Fx = A.qa + d.s2 * chunk;
@@ -1152,34 +1306,40 @@ int step_ad(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const
   A.fy2 = A.fy + d.smy * chunk;
   A.qj = A.fy2 + d.sy * chunk;
   A.qi = A.qj + d.srx * chunk;
-  // ---- forward sweep with checkpoints
+  A.cg = A.qi + d.sry * chunk;
+  // ---- forward sweep with checkpoints (six faces: halos filled before every sub-cycle)
   FV3_HIP(hipMemcpyAsync(ck, q0, sizeof(double) * qsize, hipMemcpyDeviceToDevice, s));
-  std::vector<std::vector<int>> acts(P.nmax + 1);
-  for (int it = 1; it <= P.nmax; it++) {
+  std::vector<std::vector<int>> acts(nb + 1);
+  for (int it = 1; it <= nb; it++) {
     for (int iq = 0; iq < nq; iq++)
       for (int k = 0; k < npz; k++)
-        if (P.nsplit[k] >= it) acts[it].push_back(iq * npz + k);
+        if (P0.nsplit[k] >= it) acts[it].push_back(iq * npz + k);
   }
-  for (int it = 1; it <= P.nmax; it++) {
+  for (int it = 1; it <= nb; it++) {
     double *prev = ck + (long long)(it - 1) * qsize, *next = ck + (long long)it * qsize;
+    if (faces && halo_fill<double>(c, prev, qt, nullptr, 0, nslab)) return 1;
     FV3_HIP(hipMemcpyAsync(next, prev, sizeof(double) * qsize, hipMemcpyDeviceToDevice, s));
     const std::vector<int> &act = acts[it];
     FV3_HIP(hipMemcpyAsync(zs, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, s));
-    const double *dpa = P.dp_hist + (long long)(it - 1) * npz * d.s2, *dpb = P.dp_hist + (long long)it * npz * d.s2;
-    for (size_t p0 = 0; p0 < act.size(); p0 += chunk) {
-      const int np = (int)std::min<size_t>(chunk, act.size() - p0);
-      if (subcycle_forward<double>(s, G, P.L, zs + p0, np, next, nullptr, T, dpa, dpb, ord_in, ord_ou, true)) return 1;
+    for (int t = 0; t < nt; t++) {
+      const Geo G = geo_of(c, t);
+      const double *dpa = P[t].dp_hist + (long long)(it - 1) * npz * d.s2, *dpb = P[t].dp_hist + (long long)it * npz * d.s2;
+      for (size_t p0 = 0; p0 < act.size(); p0 += chunk) {
+        const int np = (int)std::min<size_t>(chunk, act.size() - p0);
+        if (subcycle_forward<double>(s, G, P[t].L, zs + p0, np, next + t * qt, nullptr, T, dpa, dpb, ord_in, ord_ou,
+                                     true))
+          return 1;
+      }
     }
     FV3_HIP(hipStreamSynchronize(s));
   }
-  const double *qT = ck + (long long)P.nmax * qsize;
+  const double *qT = ck + (long long)nb * qsize;
   // ---- remap adjoint: qbar = (d remap / d qT)^T qbar1 on the interior
   FV3_HIP(hipMemsetAsync(qbar, 0, sizeof(double) * qsize, s));
   if (kord_tr) {
     RemapAD r{};
     r.km = npz; r.nq = nq; r.ni = d.ie - d.is + 1; r.is = d.is; r.js = d.js; r.isd = d.isd; r.jsd = d.jsd;
     r.nid = d.nid; r.kord = kord_tr; r.ncol = ncol; r.s2 = d.s2; r.ak = ak; r.bk = bk; r.ptop = ptop;
-    r.dp1 = dp1; r.qv = qT; r.gbar = qbar1; r.abar = qbar;
     r.pe1 = (double *)take(sizeof(double) * ncol * (npz + 1));
     r.pe2 = (double *)take(sizeof(double) * ncol * (npz + 1));
     r.d1 = (double *)take(sizeof(double) * ncol * (npz + 1));
@@ -1192,29 +1352,49 @@ int step_ad(Ctx &c, bool per_level, int q_split, int nq, const double *q0, const
     r.bar = (double *)take(sizeof(double) * ncol * 4 * npz);
     r.flags = (uint8_t *)take(3 * ncol * npz);
     if ((size_t)(cur - base) > bytes) return set_error("adjoint scratch layout overflow");
-    hipLaunchKernelGGL(k_remap_ad, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, s, r);
-    FV3_HIP(hipGetLastError());
+    for (int t = 0; t < nt; t++) {
+      r.dp1 = dp1 + (long long)t * d.s2 * npz;
+      r.qv = qT + t * qt; r.gbar = qbar1 + t * qt; r.abar = qbar + t * qt;
+      hipLaunchKernelGGL(k_remap_ad, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, s, r);
+      FV3_HIP(hipGetLastError());
+    }
   } else {
     if ((size_t)(cur - base) > bytes) return set_error("adjoint scratch layout overflow");
     FV3_HIP(hipMemcpyAsync(qbar, qbar1, sizeof(double) * qsize, hipMemcpyDeviceToDevice, s));
   }
-  // the adjoint lives on the interior only
-  if (box(s, [=] __device__(int i, int j, int z) {
-        const bool in = i >= d.is && i <= d.ie && j >= d.js && j <= d.je;
-        if (!in) qbar[(long long)z * d.s2 + (long long)(j - d.jsd) * d.nid + (i - d.isd)] = 0.0;
-      }, d.isd, d.ied, d.jsd, d.jed, nslab))
-    return 1;
+  // the adjoint of the step's output lives on the interior only
+  auto clear_halo = [&]() {
+    return box(s, [=] __device__(int i, int j, int z) {
+          const bool in = i >= d.is && i <= d.ie && j >= d.js && j <= d.je;
+          if (!in) qbar[(long long)z * d.s2 + (long long)(j - d.jsd) * d.nid + (i - d.isd)] = 0.0;
+        }, d.isd, d.ied, d.jsd, d.jed, nslab * nt);
+  };
+  if (clear_halo()) return 1;
   // ---- reverse sweep over sub-cycles, recomputing each from its checkpoint
-  for (int it = P.nmax; it >= 1; it--) {
+  for (int it = nb; it >= 1; it--) {
     const std::vector<int> &act = acts[it];
     FV3_HIP(hipMemcpyAsync(zs, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, s));
-    const double *dpa = P.dp_hist + (long long)(it - 1) * npz * d.s2, *dpb = P.dp_hist + (long long)it * npz * d.s2;
     double *qin = ck + (long long)(it - 1) * qsize;
-    for (size_t p0 = 0; p0 < act.size(); p0 += chunk) {
-      const int np = (int)std::min<size_t>(chunk, act.size() - p0);
-      if (subcycle_forward<double>(s, G, P.L, zs + p0, np, qin, nullptr, T, dpa, dpb, ord_in, ord_ou, false)) return 1;
-      if (sweep_weights(s, G, P.L, zs + p0, np, qin, T, W, ord_in, ord_ou)) return 1;
-      if (subcycle_adjoint(s, G, P.L, zs + p0, np, qbar, W, A, dpa, dpb)) return 1;
+    for (int t = 0; t < nt; t++) {
+      const Geo G = geo_of(c, t);
+      const double *dpa = P[t].dp_hist + (long long)(it - 1) * npz * d.s2, *dpb = P[t].dp_hist + (long long)it * npz * d.s2;
+      for (size_t p0 = 0; p0 < act.size(); p0 += chunk) {
+        const int np = (int)std::min<size_t>(chunk, act.size() - p0);
+        if (subcycle_forward<double>(s, G, P[t].L, zs + p0, np, qin + t * qt, nullptr, T, dpa, dpb, ord_in, ord_ou,
+                                     false))
+          return 1;
+        if (sweep_weights(s, G, P[t].L, zs + p0, np, qin + t * qt, T, W, ord_in, ord_ou)) return 1;
+        if (subcycle_adjoint(s, G, P[t].L, zs + p0, np, qbar + t * qt, W, A, dpa, dpb, faces)) return 1;
+      }
+    }
+    if (faces) {  // transposed halo fill: halo adjoints onto the neighbours' interiors
+      for (int s0 = 0; s0 < nslab; s0 += 65535) {
+        const int ns = std::min(65535, nslab - s0);
+        hipLaunchKernelGGL(k_halo_adjoint, dim3((unsigned)((hn + 255) / 256), (unsigned)ns), dim3(256), 0, s,
+                           qbar + (long long)s0 * d.s2, qt, d.s2, hu, hp, hd, hn);
+        FV3_HIP(hipGetLastError());
+      }
+      if (clear_halo()) return 1;
     }
     FV3_HIP(hipStreamSynchronize(s));
   }
