@@ -282,6 +282,53 @@ def run_sphere(args, torch, fa, cases):
     ctx.close()
 
 
+def run_sphere_adjoint(args, torch, fa, cases):
+    """The sphere workload's final step differentiated: six C384 faces, 127
+    levels, tracer 0 only, FP64 forward (tracer_2d_1L + kord 9 remap) with
+    its tangent-linear, then the adjoint, and the dot-product test between
+    them.  Prints one JSON line."""
+    t0 = time.perf_counter()
+    case = cases.config(4, build_q=False)
+    N, L, T = case.n, case.npz, case.ntiles
+    log(f"case 6xC{N} L{L} (tracer 0) built in {time.perf_counter() - t0:.1f}s")
+    d = device_inputs(case, torch, torch.float64)
+    nid = N + 6
+    q0 = d["q_in"].view(T, case.nq, L, nid, nid)[:, :1].contiguous().view(-1)
+    del d["q_in"], d["q_out"]
+    ctx = fa.Context(case.grids, L)
+    g = torch.Generator(device="cuda").manual_seed(7)
+
+    def field():
+        f = torch.randn(T, 1, L, nid, nid, device="cuda", dtype=torch.float64, generator=g)
+        f[..., :3, :] = 0
+        f[..., -3:, :] = 0
+        f[..., :, :3] = 0
+        f[..., :, -3:] = 0
+        return f.view(-1)
+
+    dq, y = field(), field()
+    q1, dq1, qb = torch.empty_like(q0), torch.empty_like(q0), torch.empty_like(q0)
+    delp = torch.empty_like(d["dp1"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fa.step_tl_dev(ctx, True, 0, 1, q0, dq, q1, dq1, d["dp1"], delp, d["cx"], d["cy"], d["mfx"], d["mfy"],
+                   d["ak"], d["bk"], case.ptop, case.hord, 9)
+    torch.cuda.synchronize()
+    t_tl = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fa.step_ad_dev(ctx, True, 0, 1, q0, y, qb, d["dp1"], d["cx"], d["cy"], d["mfx"], d["mfy"], d["ak"], d["bk"],
+                   case.ptop, case.hord, 9)
+    torch.cuda.synchronize()
+    t_ad = time.perf_counter() - t0
+    lhs, rhs = float(torch.dot(dq1, y)), float(torch.dot(dq, qb))
+    rel = abs(lhs - rhs) / abs(lhs)
+    log(f"TL {t_tl:.2f} s, AD {t_ad:.2f} s, <TL dx, y> {lhs:.15e} <dx, AD y> {rhs:.15e} rel {rel:.2e}")
+    ctx.close()
+    print(json.dumps({"metric": "sphere final-step adjoint", "workload": f"6xC{N} L{L}, tracer 0, FP64, "
+                      "tracer_2d_1L + kord 9 remap", "tl_s": round(t_tl, 3), "ad_s": round(t_ad, 3),
+                      "dot_product_rel": rel}), flush=True)
+
+
 def run_mixed_precision(args, torch, fa, cases):
     """One C384 tile, 64 levels, 10 tracers, deformational winds, forced
     nsplit = 2 (tracer_2d rule, q_split 2), kord 9 remap: the fused FP32 step
@@ -447,6 +494,8 @@ def main():
                          "transport (-1: one round of the launch shape; 0: off, the library's default)")
     ap.add_argument("--no-next", action="store_true", help="skip the dyn_core input timing")
     ap.add_argument("--io", action="store_true", help="also time the tracer restart write and read")
+    ap.add_argument("--sphere-adjoint", action="store_true",
+                    help="differentiate the sphere workload's final step (tracer 0, FP64 TL + AD)")
     ap.add_argument("--mixed", action="store_true",
                     help="the C384 L64 Q10 FP32-against-FP64 error report instead of the C768 tile")
     args = ap.parse_args()
@@ -467,6 +516,9 @@ def main():
         return
     if args.mixed:
         run_mixed_precision(args, torch, fa, cases)
+        return
+    if args.sphere_adjoint:
+        run_sphere_adjoint(args, torch, fa, cases)
         return
     if args.calib:  # reads 1 GiB and writes 1 GiB in one elementwise kernel
         x = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
