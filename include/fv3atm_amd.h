@@ -402,8 +402,11 @@ int fv3_history_write(fv3_ctx *ctx, const char *path, const fv3_history_args *ar
  * tracer_2d_1L (per_level = 1) followed by the tracer part of
  * Lagrangian_to_Eulerian (kord_tr; 0 leaves the remap out), with respect to
  * the interior tracer values, every limiter branch, upwind choice and
- * source-layer search frozen at the forward trajectory q0.  Winds, dp1 and
- * the single tile's halo are fixed.  Device pointers, layouts as
+ * source-layer search frozen at the forward trajectory q0.  Winds and dp1
+ * are fixed; a single tile's halo is fixed too, while on six faces every
+ * sub-cycle refreshes the halos (values and tangents) from the neighbouring
+ * faces and the adjoint carries the halo adjoints back onto the cells they
+ * were copied from.  Device pointers, layouts as
  * fv3_step_dev, all double.  The upstream dycore has no such routines; they
  * serve variational and sensitivity work on this path.
  * TL: (q0, dq0) -> (q1, dq1); q1 is the forward result, delp_out (may be
