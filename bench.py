@@ -191,10 +191,13 @@ def cpu_baseline(case, nq_all=2, nq_one=1, reps_all=5, reps_one=3):
         for r in range(reps + 1):
             t0 = time.perf_counter()
             adv = oracle.tracer_2d(c1, "1L")
-            oracle.remap_tracers(c1, adv["q"], c1.dp1, c1.pt)
+            out = oracle.remap_tracers(c1, adv["q"], c1.dp1, c1.pt)
             if r:  # the first step is the warm-up
                 ts.append(time.perf_counter() - t0)
+        last[nq] = out
         return float(np.median(ts)), case.n * case.n * case.npz * nq
+
+    last = {}
 
     omp_env = os.environ.get("OMP_NUM_THREADS")
     nthreads = L.fv3o_set_threads(int(omp_env) if omp_env and omp_env.isdigit() else os.cpu_count())
@@ -209,7 +212,7 @@ def cpu_baseline(case, nq_all=2, nq_one=1, reps_all=5, reps_one=3):
                 one_core=dict(value=c_one / t_one / 1e9, cores=1,
                               sample=f"{nq_one} tracer + pt remap, median of {reps_one} steps after one "
                                      f"warm-up, {t_one:.2f} s per step"),
-                nproc=os.cpu_count(), cpu_model=_cpu_model(), omp_num_threads=omp_env)
+                nproc=os.cpu_count(), cpu_model=_cpu_model(), omp_num_threads=omp_env), last[nq_all]
 
 
 def run_sphere(args, torch, fa, cases):
@@ -536,6 +539,7 @@ def main():
         f"{dict(zip(*np.unique(nspl, return_counts=True)))} S={S}")
     cap = int(nspl.max())
     out = {}
+    keep32 = None
 
     # ---- plain FP64 form (timing + reference for the FP32 error)
     ref64 = None
@@ -593,6 +597,7 @@ def main():
         out["fp32_mass_err_per_step"] = float(mass_err)
         log(f"FP32 vs FP64: rel L2 max {rel.max():.2e}, Linf rel max {linf.max():.2e}, "
             f"mass error {mass_err:.2e}")
+        keep32 = dict(q=d["q_out"].view(shp)[:2].double().cpu().numpy(), pt=d["pt"].view(L, nid, nid).double().cpu().numpy())
         del ref64, g32, g64
         torch.cuda.empty_cache()
         d = device_inputs(case, torch, torch.float32)
@@ -685,9 +690,24 @@ def main():
     del d
     torch.cuda.empty_cache()
     if not args.no_cpu:
-        cb = cpu_baseline(case)
+        cb, ref = cpu_baseline(case)
         log(f"cpu baseline {cb['value']:.4f} Gcell-tracer updates/s")
         out_line["cpu_baseline"] = cb
+        if keep32 is not None:
+            # the FP32 step of the workload against the FP64 oracle itself, full
+            # tile and every level, for the tracers the oracle leg ran (and pt)
+            qo, _, pto = ref
+            errs = []
+            for iq in range(qo.shape[3]):
+                a = keep32["q"][iq][:, 3:-3, 3:-3]                      # (k, j, i)
+                b = np.transpose(qo[3:-3, 3:-3, :, iq], (2, 1, 0))
+                errs.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+            a = keep32["pt"][:, 3:-3, 3:-3]
+            b = np.transpose(pto[3:-3, 3:-3, :], (2, 1, 0))
+            e_pt = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            out_line["fp32_vs_oracle"] = dict(tracers=len(errs), rel_l2=errs, pt_rel_l2=e_pt,
+                                              sample=f"full C{N} tile, {L} levels, tracers 0..{len(errs) - 1} and pt")
+            log(f"FP32 vs FP64 oracle at full size: rel L2 {max(errs):.2e}, pt {e_pt:.2e}")
     else:
         out_line["cpu_baseline"] = None
     print(json.dumps(out_line), flush=True)

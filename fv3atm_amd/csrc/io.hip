@@ -71,57 +71,64 @@ __device__ __forceinline__ double bitround(double v, int nsd) {
 __device__ __forceinline__ unsigned long long to_be(double v) { return __builtin_bswap64(__builtin_bit_cast(unsigned long long, v)); }
 __device__ __forceinline__ unsigned to_be(float v) { return __builtin_bswap32(__builtin_bit_cast(unsigned, v)); }
 
-// wave64 sum, then one atomic per wave into the field's checksum
+// block sum of the threads' checksum terms (wave shuffles, then the four
+// waves' partials in LDS) and one 64-bit atomic per block: integer sums, so
+// the result does not depend on the order
 __device__ __forceinline__ void ck_add(unsigned long long *ck, unsigned long long v) {
+  __shared__ unsigned long long part[4];
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(ck, v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(ck, part[0] + part[1] + part[2] + part[3]);
 }
 
+// blocks of the grid-stride device passes: enough to fill the device, few
+// enough that the per-block checksum atomics stay negligible
+constexpr int IO_BLOCK = 256, IO_GRID = 2048;
+dim3 io_grid(long long n) { return dim3((unsigned)std::max(1LL, std::min<long long>((n + IO_BLOCK - 1) / IO_BLOCK, IO_GRID))); }
+
 template <typename Tin, typename Tout, typename Ubits>
-__global__ void __launch_bounds__(256) k_pack(const Tin *src, Slab s, int nk, Ubits *dst, unsigned long long *ck,
-                                              int nsd) {
+__global__ void __launch_bounds__(IO_BLOCK) k_pack(const Tin *src, Slab s, int nk, Ubits *dst,
+                                                   unsigned long long *ck, int nsd) {
   const long long n = (long long)s.ni * s.nj * nk;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long c = 0;
-  if (idx < n) {
+  for (long long idx = (long long)blockIdx.x * IO_BLOCK + threadIdx.x; idx < n; idx += (long long)gridDim.x * IO_BLOCK) {
     const int i = (int)(idx % s.ni);
     const long long r = idx / s.ni;
     const int j = (int)(r % s.nj), k = (int)(r / s.nj);
     const Tout v = bitround((Tout)src[(long long)k * s.s2 + (long long)(j + s.joff) * s.nid + (i + s.ioff)], nsd);
-    c = ck_bits(v);
+    c += ck_bits(v);
     dst[idx] = to_be(v);
   }
   ck_add(ck, c);
 }
 
 template <typename Tdev>
-__global__ void __launch_bounds__(256) k_unpack(const unsigned long long *src, Slab s, int nk, Tdev *dst,
-                                                unsigned long long *ck) {
+__global__ void __launch_bounds__(IO_BLOCK) k_unpack(const unsigned long long *src, Slab s, int nk, Tdev *dst,
+                                                     unsigned long long *ck) {
   const long long n = (long long)s.ni * s.nj * nk;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long c = 0;
-  if (idx < n) {
+  for (long long idx = (long long)blockIdx.x * IO_BLOCK + threadIdx.x; idx < n; idx += (long long)gridDim.x * IO_BLOCK) {
     const int i = (int)(idx % s.ni);
     const long long r = idx / s.ni;
     const int j = (int)(r % s.nj), k = (int)(r / s.nj);
     const double v = __builtin_bit_cast(double, __builtin_bswap64(src[idx]));
-    c = ck_bits(v);
+    c += ck_bits(v);
     dst[(long long)k * s.s2 + (long long)(j + s.joff) * s.nid + (i + s.ioff)] = (Tdev)v;
   }
   ck_add(ck, c);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_checksum(const T *src, Slab s, int nk, unsigned long long *ck) {
+__global__ void __launch_bounds__(IO_BLOCK) k_checksum(const T *src, Slab s, int nk, unsigned long long *ck) {
   const long long n = (long long)s.ni * s.nj * nk;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long c = 0;
-  if (idx < n) {
+  for (long long idx = (long long)blockIdx.x * IO_BLOCK + threadIdx.x; idx < n; idx += (long long)gridDim.x * IO_BLOCK) {
     const int i = (int)(idx % s.ni);
     const long long r = idx / s.ni;
     const int j = (int)(r % s.nj), k = (int)(r / s.nj);
-    c = ck_bits(src[(long long)k * s.s2 + (long long)(j + s.joff) * s.nid + (i + s.ioff)]);
+    c += ck_bits(src[(long long)k * s.s2 + (long long)(j + s.joff) * s.nid + (i + s.ioff)]);
   }
   ck_add(ck, c);
 }
@@ -184,12 +191,12 @@ int pack_write(Ctx &c, Staging &st, FILE *fp, unsigned long long off, const Tin 
   for (int k0 = 0; k0 < nlev; k0 += kc) {
     const int nk = std::min(kc, nlev - k0);
     const long long n = (long long)sl.ni * sl.nj * nk;
-    const dim3 g((unsigned)((n + 255) / 256));
+    const dim3 g = io_grid(n);
     if (out_bytes == 8)
-      hipLaunchKernelGGL((k_pack<Tin, double, unsigned long long>), g, dim3(256), 0, s, field + k0 * sl.s2, sl, nk,
+      hipLaunchKernelGGL((k_pack<Tin, double, unsigned long long>), g, dim3(IO_BLOCK), 0, s, field + k0 * sl.s2, sl, nk,
                          (unsigned long long *)st.dev, ck, nsd);
     else
-      hipLaunchKernelGGL((k_pack<Tin, float, unsigned>), g, dim3(256), 0, s, field + k0 * sl.s2, sl, nk,
+      hipLaunchKernelGGL((k_pack<Tin, float, unsigned>), g, dim3(IO_BLOCK), 0, s, field + k0 * sl.s2, sl, nk,
                          (unsigned *)st.dev, ck, nsd);
     FV3_HIP(hipGetLastError());
     const size_t nb = (size_t)n * out_bytes;
@@ -213,7 +220,7 @@ int read_unpack(Ctx &c, Staging &st, FILE *fp, unsigned long long off, Tdev *fie
     const size_t nb = (size_t)n * 8;
     if (fread(st.host, 1, nb, fp) != nb) return set_error("the restart file ends early");
     FV3_HIP(hipMemcpyAsync(st.dev, st.host, nb, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_unpack<Tdev>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_unpack<Tdev>, io_grid(n), dim3(IO_BLOCK), 0, s,
                        (const unsigned long long *)st.dev, sl, nk, field + k0 * sl.s2, ck);
     FV3_HIP(hipGetLastError());
     FV3_HIP(hipStreamSynchronize(s));
@@ -525,11 +532,11 @@ static int field_checksum(Ctx &c, const void *field, int nslab, int elem, unsign
   const long long n = (long long)sl.ni * sl.nj * nslab;
   for (int t = 0; t < c.ntiles && n > 0; t++) {
     const long long fo = (long long)t * nslab * c.d.s2;
-    const dim3 g((unsigned)((n + 255) / 256));
+    const dim3 g = io_grid(n);
     if (elem == 8)
-      hipLaunchKernelGGL(k_checksum<double>, g, dim3(256), 0, s, (const double *)field + fo, sl, nslab, ck + t);
+      hipLaunchKernelGGL(k_checksum<double>, g, dim3(IO_BLOCK), 0, s, (const double *)field + fo, sl, nslab, ck + t);
     else
-      hipLaunchKernelGGL(k_checksum<float>, g, dim3(256), 0, s, (const float *)field + fo, sl, nslab, ck + t);
+      hipLaunchKernelGGL(k_checksum<float>, g, dim3(IO_BLOCK), 0, s, (const float *)field + fo, sl, nslab, ck + t);
     FV3_HIP(hipGetLastError());
   }
   FV3_HIP(hipMemcpyAsync(out, ck, sizeof(unsigned long long) * c.ntiles, hipMemcpyDeviceToHost, s));
