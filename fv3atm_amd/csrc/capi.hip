@@ -1026,7 +1026,7 @@ int fv3_ctx_set_launch(fv3_ctx *h, int cfg, int rows, int remap) {
   if (!h) return set_error("NULL context");
   if (cfg < 0 || cfg >= FUSED32_NCFG) return set_error("launch configuration %d out of 0..%d", cfg, FUSED32_NCFG - 1);
   if (rows < 0 || (rows > 0 && rows < 6)) return set_error("rows per block must be 0 (default) or >= 6 (got %d)", rows);
-  if (remap < 0 || remap > 1) return set_error("remap kernel %d out of 0..1", remap);
+  if (remap < 0 || remap > 2) return set_error("remap kernel %d out of 0..2", remap);
   Ctx &c = h->c;
   std::lock_guard<std::mutex> lk(c.mu);
   c.fused_cfg = cfg;

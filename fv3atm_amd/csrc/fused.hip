@@ -55,7 +55,7 @@ static int fused_step_t(Ctx &c, int nq, const T *q_in, T *q_out, T *pt, const do
     return set_error("graph-safe steps need nsplit_cap > 0 (no device-to-host read)");
   // the host-array entry points run the transport alone (c.transport_only)
   const bool remap_phase = !c.transport_only;
-  const bool fast_remap = remap_phase && std::is_same<T, float>::value && c.remap_mode == 0 && npz >= 8;
+  const bool fast_remap = remap_phase && std::is_same<T, float>::value && c.remap_mode != 1 && npz >= 8;
   // the per-column remap synchronises the host with the stream, so only the
   // column-shared FP32 remap can be captured
   if (c.graph_safe && !fast_remap)

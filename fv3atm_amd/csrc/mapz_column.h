@@ -61,7 +61,14 @@ __device__ __forceinline__ void cs_lim1(T &a1, T &a2, T &a3, T &a4, bool extm, i
       a3 = a1;
       a4 = T(0);
     } else if (fabs(a3 - a2) < -a4) {
-      if ((a1 + T(0.25) * ((a3 - a2) * (a3 - a2)) / a4 + a4 * K<T>::r12) < T(0)) {
+      // a1 + (a3-a2)^2 / (4 a4) + a4 / 12 < 0; a4 < 0 here, so the FP32 form
+      // tests the same sign multiplied through by a4, without a division
+      bool dip;
+      if constexpr (sizeof(T) == 4)
+        dip = fmaf(a1, a4, T(0.25) * ((a3 - a2) * (a3 - a2)) + a4 * a4 * K<T>::r12) > T(0);
+      else
+        dip = (a1 + T(0.25) * ((a3 - a2) * (a3 - a2)) / a4 + a4 * K<T>::r12) < T(0);
+      if (dip) {
         if (a1 < a3 && a1 < a2) {
           a3 = a1;
           a2 = a1;

@@ -482,6 +482,424 @@ __global__ void __launch_bounds__(RF_COLS * RF_WAVES) k_remap_fields(const Field
   }
 }
 
+
+// ---- levels across lanes ----------------------------------------------------
+// k_remap_rows<M, L>: a wave holds 64/L columns, L lanes each (one DPP row for
+// L = 16, two for L = 32); lane g of a column holds its levels g*M+1 ..
+// g*M+M (those cells, and the interface at the top of each).  The column
+// data (elimination factors and the target cut list) stay in registers while
+// the wave walks every field of its columns, so each field is read from HBM
+// once:
+//   - the interface solve is the same Thomas recurrence, run inside each lane
+//     from a zero entry value, composed across the column by an affine scan
+//     (the slopes are field independent and scanned once per column), and run
+//     again inside each lane from the true entry value;
+//   - the interface limits, extremum flags and cell limiters are evaluated per
+//     level with the neighbouring lanes' values moved in by DPP; the first two
+//     and last two cells and interfaces take their own forms;
+//   - each cell's parabola and mass go to LDS, and each lane integrates its
+//     Eulerian layers from the parabolas of the source layers that hold their
+//     top and bottom interfaces, plus the masses of the layers in between.
+// The pieces of each Eulerian layer are summed in the order of the
+// one-thread-per-(column, field) kernel above (bottom piece, whole layers
+// upward, top piece).
+constexpr int RW_WAVES = 8;  // waves per workgroup
+
+struct RowsArgs {
+  int km, ni, is, js, isd, jsd, nid, nq, nf, kord_tr, kord_tm;
+  unsigned nwg;
+  long long ncol, s2;
+  float t_min;
+  const float *q_src, *pt_src;
+  float *q_dst, *pt_dst;
+  RemapCols cols;
+};
+
+// Cross-lane moves.  Each result is pinned by an empty volatile asm where it
+// is computed: a DPP or bpermute that the compiler moved into a lane-dependent
+// branch would run with some source lanes disabled and read 0 from them.
+__device__ __forceinline__ int pin(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp0(int v) {  // 0 where the source lane is invalid
+  return pin(__builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp0f(float v) {
+  return __builtin_bit_cast(float, dpp0<CTRL>(__builtin_bit_cast(int, v)));
+}
+// lane l reads lane l-N (l+N) of its 16-lane row; 0 past the row's end
+template <int N>
+__device__ __forceinline__ float row_lo(float v) { return dpp0f<0x110 + N>(v); }
+template <int N>
+__device__ __forceinline__ float row_hi(float v) { return dpp0f<0x100 + N>(v); }
+// the previous / next lane of the column: within the row (L = 16), or across
+// the whole wave (L = 32; at a column boundary it reads the other column's
+// lane, whose value is never used there); 0 past the wave's end
+template <int L>
+__device__ __forceinline__ int nb_lo_i(int v) { return dpp0<L == 16 ? 0x111 : 0x138>(v); }
+template <int L>
+__device__ __forceinline__ int nb_hi_i(int v) { return dpp0<L == 16 ? 0x101 : 0x130>(v); }
+template <int L>
+__device__ __forceinline__ float nb_lo(float v) { return __builtin_bit_cast(float, nb_lo_i<L>(__builtin_bit_cast(int, v))); }
+template <int L>
+__device__ __forceinline__ float nb_hi(float v) { return __builtin_bit_cast(float, nb_hi_i<L>(__builtin_bit_cast(int, v))); }
+
+// inclusive affine scans over the lanes of a column: lane g holds the map
+// y -> s y + v of its levels; after the scan v is the map's value for a zero
+// entry at the column's end (lo: from lane 0 upward; hi: from lane L-1
+// downward).  s[j] are the lanes' slope products at each step.
+template <int L>
+__device__ __forceinline__ float scan_lo(float v, const float *s) {
+  v = fmaf(s[0], row_lo<1>(v), v);
+  v = fmaf(s[1], row_lo<2>(v), v);
+  v = fmaf(s[2], row_lo<4>(v), v);
+  v = fmaf(s[3], row_lo<8>(v), v);
+  if (L == 32)  // the second row of each column takes the first row's last lane (row_bcast:15)
+    v = fmaf(s[4], __builtin_bit_cast(float, dpp0<0x142, 0xa>(__builtin_bit_cast(int, v))), v);
+  return v;
+}
+template <int L>
+__device__ __forceinline__ float scan_hi(float v, const float *s, int lane) {
+  v = fmaf(s[0], row_hi<1>(v), v);
+  v = fmaf(s[1], row_hi<2>(v), v);
+  v = fmaf(s[2], row_hi<4>(v), v);
+  v = fmaf(s[3], row_hi<8>(v), v);
+  if (L == 32) {  // the first row of each column takes the second row's first lane
+    const float u = __builtin_bit_cast(float, pin(__builtin_amdgcn_ds_bpermute(((lane & 32) | 16) << 2, __builtin_bit_cast(int, v))));
+    v = (lane & 16) == 0 ? fmaf(s[4], u, v) : v;
+  }
+  return v;
+}
+// slope products: a lane with no partner at a step keeps its product (the
+// second row of a column needs its own row's product at the cross-row step)
+template <int CTRL>
+__device__ __forceinline__ float slope_mv(float v) {
+  return __builtin_bit_cast(float, pin(__builtin_amdgcn_update_dpp(__builtin_bit_cast(int, 1.f), __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false)));
+}
+__device__ __forceinline__ void slopes_lo(float sl, float *s) {
+  s[0] = sl; sl *= slope_mv<0x111>(sl);
+  s[1] = sl; sl *= slope_mv<0x112>(sl);
+  s[2] = sl; sl *= slope_mv<0x114>(sl);
+  s[3] = sl; sl *= slope_mv<0x118>(sl);
+  s[4] = sl;
+}
+__device__ __forceinline__ void slopes_hi(float sl, float *s) {
+  s[0] = sl; sl *= slope_mv<0x101>(sl);
+  s[1] = sl; sl *= slope_mv<0x102>(sl);
+  s[2] = sl; sl *= slope_mv<0x104>(sl);
+  s[3] = sl; sl *= slope_mv<0x108>(sl);
+  s[4] = sl;
+}
+
+// staging tile of a field: [column of the workgroup][level], the level index
+// skewed by k >> 5 and the row padded to an odd length, so that the
+// coalesced writes (consecutive columns) and the per-lane reads (M levels
+// 4 apart across lanes) spread over the LDS banks
+template <int LM>
+struct Stage {
+  static constexpr int row = LM + LM / 32 + 1 + ((LM + LM / 32 + 1) % 2 == 0 ? 1 : 0);
+  static __device__ __forceinline__ int at(int c, int k) { return c * row + k + (k >> 5); }
+};
+
+// cell records: [column][level], the level index k stored at k + (k >> 2),
+// so that the lanes' 16-byte writes (4 levels apart) fall on distinct bank
+// groups for 8 or 16 lanes at a time
+template <int LM>
+struct Recs {
+  static constexpr int row = LM + LM / 4;
+  static __device__ __forceinline__ int at(int c, int k) { return c * row + k + (k >> 2); }
+};
+
+template <int M, int L>
+size_t rows_lds_bytes() {
+  constexpr int wgc = RW_WAVES * (64 / L);
+  return sizeof(float4) * wgc * Recs<L * M>::row + 2 * sizeof(float) * wgc * Stage<L * M>::row;
+}
+
+template <int M, int L>
+__global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs a) {
+  constexpr int CPW = 64 / L;              // columns per wave
+  constexpr int WGC = RW_WAVES * CPW;      // columns per workgroup
+  constexpr int NT = 64 * RW_WAVES;        // threads per workgroup
+  constexpr int KS = NT / WGC;             // level step of a thread's staged elements
+  using ST = Stage<L * M>;
+  using RC = Recs<L * M>;
+  // LDS: cell records (c0 c1 c2 mass) of the workgroup's columns, then the
+  // field's input and output staging tiles
+  extern __shared__ float4 rw_lds[];
+  float *const s_in = reinterpret_cast<float *>(rw_lds + WGC * RC::row);
+  float *const s_out = s_in + WGC * ST::row;
+  const int tid = threadIdx.x, lane = tid & 63, cw = lane / L;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // blocks are dealt round-robin to the 8 XCDs: renumber so that neighbouring
+  // column blocks (which share cache lines) run on one XCD
+  const unsigned bid = blockIdx.x, nx = 8, q8 = a.nwg / nx, r8 = a.nwg % nx;
+  const unsigned xcd = bid % nx, idx = bid / nx;
+  const unsigned logical = xcd < r8 ? xcd * (q8 + 1) + idx : r8 * (q8 + 1) + (xcd - r8) * q8 + idx;
+  const long long nc = a.ncol;
+  const long long wcol0 = (long long)logical * WGC;  // the workgroup's first column
+  // (columns past the last one compute on a copy of it and store nothing;
+  // every wave takes part in the workgroup barriers)
+  const int wc = wave * CPW + cw;  // the lane's column within the workgroup
+  const long long col = std::min(wcol0 + wc, nc - 1);
+  const int km = a.km;
+  const int gc = lane & (L - 1);
+  const int k0c = gc * M;  // the lane's levels are k0+1 .. k0+M
+  const unsigned s2b = (unsigned)a.s2 * 4u, fbytes = (unsigned)km * s2b;
+  // the thread's staged elements: column tc of the workgroup, levels
+  // tk0 + j KS (j < M), read and written with consecutive columns across
+  // lanes; levels past km fall past the buffer end, where the hardware reads 0
+  // and drops stores; stores of columns past the last go there too
+  const int tc = tid % WGC, tk0 = tid / WGC;
+  unsigned vtc, vts;
+  {
+    const bool t_act = wcol0 + tc < nc;
+    const long long tcol = t_act ? wcol0 + tc : nc - 1;
+    const int ti = (int)(tcol % a.ni), tj = (int)(tcol / a.ni);
+    const unsigned tcell = (unsigned)((tj + a.js - a.jsd) * a.nid + (ti + a.is - a.isd));
+    vtc = (unsigned)tk0 * s2b + tcell * 4u;
+    vts = t_act ? vtc : 0x80000000u;
+  }
+  const unsigned kstep = (unsigned)KS * s2b;
+
+  // ---- column data (levels past km read 0)
+  const unsigned lvl = (unsigned)nc * 4u, cbytes = (unsigned)km * lvl;
+  float d4[M], rbt[M], gg[M], dp1[M], st[M], wt[M], wb[M], rd[M];
+  int ltc[M];
+  {
+    const Rsrc r_d4 = make_rsrc(a.cols.d4, cbytes), r_rbt = make_rsrc(a.cols.rbt, cbytes);
+    const Rsrc r_gg = make_rsrc(a.cols.gg, cbytes), r_dp1 = make_rsrc(a.cols.dp1, cbytes);
+    const Rsrc r_lk = make_rsrc(a.cols.lk, cbytes), r_cut = make_rsrc(a.cols.cut, 4u * cbytes);
+    const unsigned vc = (unsigned)k0c * lvl + (unsigned)col * 4u;
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      const unsigned o = vc + (unsigned)i * lvl;
+      d4[i] = ldf(r_d4, o, 0);
+      rbt[i] = ldf(r_rbt, o, 0);
+      gg[i] = ldf(r_gg, o, 0);
+      dp1[i] = ldf(r_dp1, o, 0);
+      ltc[i] = ldi(r_lk, o, 0);
+      st[i] = ldf(r_cut, 4u * o, 0);
+      wt[i] = ldf(r_cut, 4u * o + 4u, 0);
+      wb[i] = ldf(r_cut, 4u * o + 8u, 0);
+      rd[i] = ldf(r_cut, 4u * o + 12u, 0);
+      // interface km+1 (and the padding past it) lies at the bottom of layer km
+      if (k0c + i + 1 > km) {
+        ltc[i] = km;
+        st[i] = 1.f;
+      }
+    }
+  }
+  const float bot0 = a.cols.bot[col], bot1 = a.cols.bot[nc + col], bot2 = a.cols.bot[2 * nc + col];
+  // the source layer and fraction of the bottom interface of the lane's last
+  // target (the last lane's last target is padding: km+1 <= L M)
+  // (the moves run for every lane before the selection: the last lane's value
+  // is read by its neighbour)
+  const int ltn_n = nb_hi_i<L>(ltc[0]);
+  const float stn_n = nb_hi<L>(st[0]);
+  const int ltn = gc == L - 1 ? km : ltn_n;
+  const float stn = gc == L - 1 ? 1.f : stn_n;
+  // slopes of the lane's forward (Q'(k0) -> Q'(k0+M)) and backward
+  // (Q(k0+M+1) -> Q(k0+1)) maps, and their products at the scan steps
+  float fs[5], bs[5];
+  {
+    float sf = 1.f, sb = 1.f;
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      sf *= -rbt[i];
+      sb *= -gg[i];
+    }
+    slopes_lo(sf, fs);
+    slopes_hi(sb, bs);
+  }
+  // wave-uniform positions of the special levels
+  const int g_km = (km - 1) / M, s_km = (km - 1) % M;      // cell / interface km
+  const int g_km1 = (km - 2) / M, s_km1 = (km - 2) % M;    // cell km-1
+  const int g_kb = km / M, s_kb = km % M;                  // interface km+1
+
+  float qn[M];  // the thread's staged elements of the next field
+  {
+    const Rsrc r0 = make_rsrc(a.nq > 0 ? a.q_src : a.pt_src, fbytes);
+#pragma unroll
+    for (int j = 0; j < M; j++) qn[j] = ldf(r0, vtc + (unsigned)j * kstep, 0);
+  }
+  for (int f = 0; f < a.nf; f++) {
+    // level-dependent offsets, masks and LDS addresses are cheaper to
+    // recompute every field than to keep live across the loop
+    int k0 = k0c, g = gc, tk = tk0, tcl = tc;
+    unsigned vt = vtc, vs = vts;
+    int lt[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) lt[i] = ltc[i];
+    asm volatile("" : "+v"(k0), "+v"(g), "+v"(tk), "+v"(tcl), "+v"(vt), "+v"(vs));
+#pragma unroll
+    for (int i = 0; i < M; i++) asm volatile("" : "+v"(lt[i]));
+    auto rec_of = [&](int l) -> float4 & { return rw_lds[RC::at(wc, l - 1)]; };
+
+    const bool is_pt = f >= a.nq;
+    const int iv = is_pt ? 1 : 0;
+    const int akord = is_pt ? a.kord_tm : a.kord_tr;
+    const float qmin = is_pt ? a.t_min : 0.f;
+    const bool lim = akord <= 16;
+    // stage the field; then each lane takes its column's levels
+#pragma unroll
+    for (int j = 0; j < M; j++) s_in[ST::at(tcl, tk + j * KS)] = qn[j];
+    __syncthreads();
+    if (f + 1 < a.nf) {  // next field's elements, in flight while this one is computed
+      const int fn = f + 1;
+      const Rsrc rn = make_rsrc(fn >= a.nq ? a.pt_src : a.q_src + (long long)fn * km * a.s2, fbytes);
+#pragma unroll
+      for (int j = 0; j < M; j++) qn[j] = ldf(rn, vt + (unsigned)j * kstep, 0);
+    }
+    const Rsrc rdst = make_rsrc(is_pt ? a.pt_dst : a.q_dst + (long long)f * km * a.s2, fbytes);
+    float q[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) q[i] = s_in[ST::at(wc, k0 + i)];
+
+    // cell means of levels k0-1 .. k0+M+2 (A(j) is level k0+j+1)
+    const float am2 = nb_lo<L>(q[M - 2]), am1 = nb_lo<L>(q[M - 1]);
+    const float ap0 = nb_hi<L>(q[0]), ap1 = nb_hi<L>(q[1]);
+    auto A = [&](int j) -> float { return j == -2 ? am2 : j == -1 ? am1 : j == M ? ap0 : j == M + 1 ? ap1 : q[j]; };
+
+    // ---- forward elimination: Q'(k) = (3 (a(k-1) + d4 a(k)) - Q'(k-1)) rbt;
+    // level 1: Q'(1) = (d4 a(1) + a(2)) rbt
+    const float t1 = fmaf(d4[0], q[0], q[1]);
+    float x[M];
+    {
+      float e = (g == 0 ? t1 : 3.f * fmaf(d4[0], q[0], am1)) * rbt[0];
+#pragma unroll
+      for (int i = 1; i < M; i++) e = (3.f * fmaf(d4[i], q[i], q[i - 1]) - e) * rbt[i];
+      const float cin = nb_lo<L>(scan_lo<L>(e, fs));
+      e = g == 0 ? t1 * rbt[0] : (3.f * fmaf(d4[0], q[0], am1) - cin) * rbt[0];
+      x[0] = e;
+#pragma unroll
+      for (int i = 1; i < M; i++) x[i] = e = (3.f * fmaf(d4[i], q[i], q[i - 1]) - e) * rbt[i];
+    }
+    // bottom row: Q(km+1) from a(km), a(km-1) and Q'(km), placed at interface km+1
+    {
+      float a_km = 0.f, a_km1 = 0.f, x_km = 0.f;
+#pragma unroll
+      for (int i = 0; i < M; i++)
+        if (i == s_km) {
+          a_km = q[i];
+          a_km1 = A(i - 1);
+          x_km = x[i];
+        }
+      float qb = (bot0 * a_km + a_km1 - bot1 * x_km) * bot2;
+      if (s_kb == 0) qb = nb_lo<L>(qb);
+#pragma unroll
+      for (int i = 0; i < M; i++)
+        if (i == s_kb && g == g_kb) x[i] = qb;
+    }
+    // ---- back substitution: Q(k) = Q'(k) - gg Q(k+1) (gg is 0 from km+1 on)
+    {
+      float e = 0.f;
+#pragma unroll
+      for (int i = M - 1; i >= 0; i--) e = fmaf(-gg[i], e, x[i]);
+      e = nb_hi<L>(scan_hi<L>(e, bs, lane));
+#pragma unroll
+      for (int i = M - 1; i >= 0; i--) x[i] = e = fmaf(-gg[i], e, x[i]);
+    }
+
+    // ---- interface limits, in place (interface p = k0+i+1 lies between cells
+    // p-1 and p); interfaces 1 and km+1 take no limit, 2 and km the two-sided
+    // one (their slots are known to the whole wave: only there do the lanes
+    // that hold them take their own forms)
+    if (lim) {
+      const float dn_floor = iv == 0 ? 0.f : -INFINITY;
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        const float am = A(i - 1), a0 = A(i);
+        const float lo = fminf(am, a0), hi = fmaxf(am, a0);
+        const float glo = am - A(i - 2), ghi = A(i + 1) - a0;
+        const float both = fmaxf(fminf(x[i], hi), lo);
+        const float up = fmaxf(x[i], lo);
+        const float dn = fmaxf(fminf(x[i], hi), dn_floor);
+        float v = glo * ghi > 0.f ? both : (glo > 0.f ? up : dn);
+        if (i == 0 && g == 0) v = x[i];
+        if (i == 1 && g == 0) v = both;
+        if (i == s_km && g == g_km) v = both;
+        if (i == s_kb && g == g_kb) v = x[i];
+        x[i] = v;
+      }
+    }
+    const float qln = nb_hi<L>(x[0]);  // interface k0+M+1
+    auto QL = [&](int j) -> float { return j == M ? qln : x[j]; };
+
+    // ---- profile of every cell (a2 = Ql(k), a3 = Ql(k+1), a4), staged in LDS
+    // with the cell's mass
+    if (lim) {
+      unsigned eb = 0;  // 3 flag bits per cell k0+i+1: 1 extm, 2 ext5, 4 ext6
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        const float a0 = A(i);
+        bool em = (a0 - A(i - 1)) * (A(i + 1) - a0) < 0.f;
+        // cells 1 and km: extremum against their own interface values
+        if ((i == 0 && g == 0) || (i == s_km && g == g_km)) em = (QL(i) - a0) * (QL(i + 1) - a0) > 0.f;
+        unsigned e = em ? 1u : 0u;
+        if (akord > 9) {
+          const float x0 = 2.f * a0 - (QL(i) + QL(i + 1)), x1 = fabsf(QL(i) - QL(i + 1));
+          e |= (fabsf(x0) > x1 ? 2u : 0u) | (fabsf(3.f * x0) > x1 ? 4u : 0u);
+        }
+        eb |= e << (3 * i);
+      }
+      const int e_prev = (int)((unsigned)nb_lo_i<L>((int)eb) >> (3 * (M - 1))) & 7;
+      const int e_next = nb_hi_i<L>((int)eb) & 7;
+      auto EF = [&](int j) -> int { return j < 0 ? e_prev : j >= M ? e_next : (int)(eb >> (3 * j)) & 7; };
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        float a1 = q[i], a2 = QL(i), a3 = QL(i + 1), a4;
+        // the interior form (3 stands for any cell of 3 .. km-2); cells 1, 2,
+        // km-1 and km in their own forms
+        int c = 3;
+        if (i == 1 && g == 0) c = 2;
+        if (i == s_km1 && g == g_km1) c = km - 1;
+        if (i == s_km && g == g_km) c = km;
+        if (i == 0 && g == 0) {
+          if (iv == 0) a2 = fmaxf(0.f, a2);
+          a4 = 3.f * (2.f * a1 - (a2 + a3));
+          cs_lim1<float>(a1, a2, a3, a4, (EF(0) & 1) != 0, 1);
+        } else {
+          cell_limit(c, km, iv, akord, qmin, a1, a2, a3, a4, EF(i - 1), EF(i), EF(i + 1), A(i - 1) - A(i - 2),
+                     A(i) - A(i - 1), A(i + 1) - A(i), A(i + 2) - A(i + 1));
+        }
+        rec_of(k0 + i + 1) = make_float4(a2, 0.5f * (a4 + a3 - a2), a4 * (1.0f / 3.0f), dp1[i] * q[i]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        const float a2 = QL(i), a3 = QL(i + 1), a4 = 3.f * (2.f * q[i] - (a2 + a3));
+        rec_of(k0 + i + 1) = make_float4(a2, 0.5f * (a4 + a3 - a2), a4 * (1.0f / 3.0f), dp1[i] * q[i]);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    // ---- Eulerian layers k0+1 .. k0+M
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      const int l_t = lt[i], l_b = i + 1 < M ? lt[i + 1] : ltn;
+      const float s_t = st[i], s_b = i + 1 < M ? st[i + 1] : stn;
+      const float4 ct = rec_of(l_t), cb = rec_of(l_b);
+      // within one source layer: the parabola's mean over [s_t, s_b]; else
+      // the bottom piece, the whole layers between, and the top piece over
+      // [s_t, 1], weighted by their pressure widths
+      const bool same = l_t == l_b;
+      const float top = pmean(ct.x, ct.y, ct.z, s_t, same ? s_b : 1.f);
+      float acc = wb[i] * pmean(cb.x, cb.y, cb.z, 0.f, s_b);
+      for (int l = l_b - 1; l > l_t; l--) acc += rec_of(l).w;
+      s_out[ST::at(wc, k0 + i)] = same ? top : (acc + wt[i] * top) * rd[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < M; j++) stf(rdst, vs + (unsigned)j * kstep, s_out[ST::at(tcl, tk + j * KS)]);
+  }
+}
+
 }  // namespace
 
 size_t remap_cols_bytes(int km, long long ncol) {
@@ -523,6 +941,13 @@ static int check_sizes(int km, long long ncol, long long s2) {
     return set_error("fast remap: a tile's column data or field level set exceeds 2 GiB "
                      "(the one-thread-per-column remap, fv3_ctx_set_launch remap = 1, has no such limit)");
   return 0;
+}
+
+// the levels-across-lanes kernel addresses up to 128 levels (the padding past
+// km included) of a field and of the column data with 32-bit byte offsets
+static bool rows_fits(int km, long long ncol, long long s2) {
+  (void)km;
+  return 4LL * 128 * s2 < (1LL << 31) && 16LL * 128 * ncol < (1LL << 31);
 }
 
 static int check_orders(int km, int kord_tr, int kord_tm, bool with_pt) {
@@ -573,6 +998,28 @@ int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_d
     fa.o_lk = (unsigned)((const char *)cols.lk - b);
     fa.b_cols = (unsigned)span;
     fa.b_cut = (unsigned)cut;
+  }
+  if (c.remap_mode == 0 && km + 1 <= 128 && rows_fits(km, ncol, d.s2)) {
+    RowsArgs ra{};
+    ra.km = km; ra.ni = ni; ra.is = d.is; ra.js = d.js; ra.isd = d.isd; ra.jsd = d.jsd; ra.nid = d.nid;
+    ra.nq = nq; ra.nf = fa.nf; ra.kord_tr = kord_tr; ra.kord_tm = kord_tm;
+    ra.ncol = ncol; ra.s2 = d.s2; ra.t_min = (float)t_min;
+    ra.q_src = q_src; ra.pt_src = pt_src; ra.q_dst = q_dst; ra.pt_dst = pt_dst; ra.cols = cols;
+    // levels per lane x lanes per column: 2 x 16, 4 x 16 or 4 x 32
+    const int M = km + 1 <= 32 ? 2 : 4, L = km + 1 <= 64 ? 16 : 32;
+    const long long per_wg = (long long)RW_WAVES * (64 / L);
+    const long long nwg = (ncol + per_wg - 1) / per_wg;
+    if (nwg > 0x7fffffffLL) return set_error("remap grid too large");
+    ra.nwg = (unsigned)nwg;
+    const size_t lds = M == 2 ? rows_lds_bytes<2, 16>() : L == 16 ? rows_lds_bytes<4, 16>() : rows_lds_bytes<4, 32>();
+    if (M == 2)
+      hipLaunchKernelGGL((k_remap_rows<2, 16>), dim3((unsigned)nwg), dim3(64 * RW_WAVES), lds, s, ra);
+    else if (L == 16)
+      hipLaunchKernelGGL((k_remap_rows<4, 16>), dim3((unsigned)nwg), dim3(64 * RW_WAVES), lds, s, ra);
+    else
+      hipLaunchKernelGGL((k_remap_rows<4, 32>), dim3((unsigned)nwg), dim3(64 * RW_WAVES), lds, s, ra);
+    FV3_HIP(hipGetLastError());
+    return 0;
   }
   const long long ncb = (ncol + RF_COLS - 1) / RF_COLS;
   const long long nwg = ncb * fa.nfg;

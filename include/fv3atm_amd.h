@@ -169,9 +169,11 @@ int fv3_ctx_set_timing(fv3_ctx *ctx, int on);
  * (0..14; the default is 9: 4 waves x 2 tracers for the interior kernel, 4 x 1
  * at three workgroups per CU for the edge kernels); rows the interior rows
  * per block (0 = 60);
- * remap the vertical remap kernel, 0 = column-shared fast kernel (default),
- * 1 = one thread per column.  Transport results do not depend on cfg/rows;
- * the two remap kernels agree to FP32 rounding. */
+ * remap the vertical remap kernel: 0 (default) = column-shared, a column's
+ * levels spread over the 16 lanes of a DPP row (npz <= 127; larger npz take
+ * kernel 2), 1 = one thread per column, 2 = column-shared, one thread per
+ * (column, field).  Transport results do not depend on cfg/rows; the remap
+ * kernels agree to FP32 rounding. */
 int fv3_ctx_set_launch(fv3_ctx *ctx, int cfg, int rows, int remap);
 /* No counterpart upstream: run the fused transport's two edge kernels (the
  * blocks that reach a halo or cube-edge row or column) beside the lean

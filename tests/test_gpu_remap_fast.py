@@ -1,5 +1,7 @@
-"""GPU checks of the FP32 step's column-shared remap kernel (remap kernel 0)
-against the one-thread-per-column FP32 remap (kernel 1), the FP64 oracle,
+"""GPU checks of the FP32 step's column-shared remap kernels (remap kernel 0:
+a column's levels spread over the lanes of a wave; kernel 2: one thread per
+(column, field)) against the one-thread-per-column FP32 remap (kernel 1),
+the FP64 oracle,
 and what the mathematics fixes: means of a profile linear in pressure map to
 the means of the same line over the target layers, constants stay constant,
 column mass is conserved.  Layer thicknesses are perturbed by up to 45 % per
@@ -98,6 +100,32 @@ def test_fast_remap_matches_column_remap_and_oracle(gpu_ready, kord_tr, kord_tm)
         assert e_col <= tol and e_64 <= tol
     a, r = interior(fast["pt"]), interior(pt64)
     print(f"pt: rel L2 fast-fp64 {rel_l2(a, r):.2e}")
+    assert rel_l2(a, r) <= 1e-6
+
+
+# 8, 32, 63: 2 or 4 levels on each of 16 lanes (32: interface km+1 opens a
+# lane); 64, 127: 4 levels on each of 32 lanes.  18 x 18 columns leave a
+# partly filled last workgroup.
+@pytest.mark.parametrize("npz", [8, 32, 63, 64, 127])
+@pytest.mark.parametrize("kord_tr,kord_tm", [(9, 10), (10, 9), (8, 8), (22, 22)])
+def test_lane_remap_layouts(gpu_ready, npz, kord_tr, kord_tm):
+    case = structured_case(courant=0.6, seed=11, n=18, npz=npz)
+    print("case", case.name, "kord", kord_tr, kord_tm, flush=True)
+    lanes = run32(case, 0, kord_tr, kord_tm)
+    per_field = run32(case, 2, kord_tr, kord_tm)
+    adv = oracle.tracer_2d(case, "1L")
+    q64, _, pt64 = oracle.remap_tracers(case, adv["q"], case.dp1, case.pt, kord_tr=kord_tr, kord_tm=kord_tm)
+    sens = oracle_sensitivity(case, adv["q"], kord_tr, kord_tm, q64)
+    np.testing.assert_array_equal(interior(lanes["delp"]), interior(per_field["delp"]))
+    for iq in range(case.nq):
+        a, b, r = (interior(x[..., iq]) for x in (lanes["q"], per_field["q"], q64))
+        e_pf, e_64 = rel_l2(a, b), rel_l2(a, r)
+        print(f"tracer {iq}: rel L2 lanes-per_field {e_pf:.2e} lanes-fp64 {e_64:.2e} "
+              f"per_field-fp64 {rel_l2(b, r):.2e} sensitivity {sens[iq]:.2e}")
+        tol = max(1e-5, 4.0 * sens[iq])
+        assert e_pf <= tol and e_64 <= tol
+    a, r = interior(lanes["pt"]), interior(pt64)
+    print(f"pt: rel L2 lanes-fp64 {rel_l2(a, r):.2e}")
     assert rel_l2(a, r) <= 1e-6
 
 
