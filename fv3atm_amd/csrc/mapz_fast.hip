@@ -198,8 +198,44 @@ __device__ __forceinline__ TargetRow load_target(const ColBuf &cb, unsigned tvo)
   return t;
 }
 
-// Limited cell coefficients of source layer c (2 <= c <= km) from the window.
+// Limited cell coefficients of an interior source layer (3 <= c <= km-2).
 // e_lo/e_c/e_hi: flag bits (1 extm, 2 ext5, 4 ext6) of cells c-1, c, c+1.
+__device__ __forceinline__ void cell_limit_interior(int iv, int akord, float qmin, float &a1, float &a2, float &a3,
+                                                    float &a4, int e_lo, int e_c, int e_hi, float gkm1, float gk,
+                                                    float gk1, float gk2) {
+  if (akord < 9) {
+    huynh2f(a1, a2, a3, gk, gk1, gk2, gkm1);
+    a4 = 3.f * (2.f * a1 - (a2 + a3));
+  } else if (akord == 9) {
+    const bool em = (e_c & 1) != 0;
+    if ((em && (e_lo & 1)) || (em && (e_hi & 1)) || (em && a1 < qmin)) {
+      a2 = a1;
+      a3 = a1;
+      a4 = 0.f;
+    } else {
+      a4 = 6.f * a1 - 3.f * (a2 + a3);
+      if (fabsf(a4) > fabsf(a2 - a3)) {
+        huynh2f(a1, a2, a3, gk, gk1, gk2, gkm1);
+        a4 = 6.f * a1 - 3.f * (a2 + a3);
+      }
+    }
+  } else {
+    if (e_c & 2) {
+      if ((e_lo & 2) || (e_hi & 2) || a1 < qmin) {
+        a2 = a1;
+        a3 = a1;
+      } else if ((e_lo & 4) || (e_hi & 4)) {
+        huynh2f(a1, a2, a3, gk, gk1, gk2, gkm1);
+      }
+    } else if (e_c & 4) {
+      if ((e_lo & 2) || (e_hi & 2)) huynh2f(a1, a2, a3, gk, gk1, gk2, gkm1);
+    }
+    a4 = 3.f * (2.f * a1 - (a2 + a3));
+  }
+  if (iv == 0) cs_lim1<float>(a1, a2, a3, a4, (e_c & 1) != 0, 0);
+}
+
+// Limited cell coefficients of source layer c (2 <= c <= km) from the window.
 __device__ __forceinline__ void cell_limit(int c, int km, int iv, int akord, float qmin, float &a1,
                                            float &a2, float &a3, float &a4, int e_lo, int e_c,
                                            int e_hi, float gkm1, float gk, float gk1, float gk2) {
@@ -215,36 +251,7 @@ __device__ __forceinline__ void cell_limit(int c, int km, int iv, int akord, flo
     a4 = 3.f * (2.f * a1 - (a2 + a3));
     cs_lim1<float>(a1, a2, a3, a4, (e_c & 1) != 0, 2);
   } else {
-    if (akord < 9) {
-      huynh2f(a1, a2, a3, gk, gk1, gk2, gkm1);
-      a4 = 3.f * (2.f * a1 - (a2 + a3));
-    } else if (akord == 9) {
-      const bool em = (e_c & 1) != 0;
-      if ((em && (e_lo & 1)) || (em && (e_hi & 1)) || (em && a1 < qmin)) {
-        a2 = a1;
-        a3 = a1;
-        a4 = 0.f;
-      } else {
-        a4 = 6.f * a1 - 3.f * (a2 + a3);
-        if (fabsf(a4) > fabsf(a2 - a3)) {
-          huynh2f(a1, a2, a3, gk, gk1, gk2, gkm1);
-          a4 = 6.f * a1 - 3.f * (a2 + a3);
-        }
-      }
-    } else {
-      if (e_c & 2) {
-        if ((e_lo & 2) || (e_hi & 2) || a1 < qmin) {
-          a2 = a1;
-          a3 = a1;
-        } else if ((e_lo & 4) || (e_hi & 4)) {
-          huynh2f(a1, a2, a3, gk, gk1, gk2, gkm1);
-        }
-      } else if (e_c & 4) {
-        if ((e_lo & 2) || (e_hi & 2)) huynh2f(a1, a2, a3, gk, gk1, gk2, gkm1);
-      }
-      a4 = 3.f * (2.f * a1 - (a2 + a3));
-    }
-    if (iv == 0) cs_lim1<float>(a1, a2, a3, a4, (e_c & 1) != 0, 0);
+    cell_limit_interior(iv, akord, qmin, a1, a2, a3, a4, e_lo, e_c, e_hi, gkm1, gk, gk1, gk2);
   }
 }
 
@@ -664,6 +671,12 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
     vts = t_act ? vtc : 0x80000000u;
   }
   const unsigned kstep = (unsigned)KS * s2b;
+  // LDS bases: the thread's staged elements sit at compile-time offsets
+  // from sbt (its levels are tk0 + j KS with tk0 < KS <= 32), the lane's
+  // levels k0 + i at offsets i from slr and srec (k0 is a multiple of M and
+  // M <= 4, so neither skew changes across them)
+  static_assert(KS <= 32 && (KS & (KS - 1)) == 0 && M <= 4, "staging offsets");
+  const int sbt = ST::at(tc, tk0), slr = ST::at(wc, k0c), srec = RC::at(wc, k0c);
 
   // ---- column data (levels past km read 0)
   const unsigned lvl = (unsigned)nc * 4u, cbytes = (unsigned)km * lvl;
@@ -729,15 +742,15 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
   for (int f = 0; f < a.nf; f++) {
     // level-dependent offsets, masks and LDS addresses are cheaper to
     // recompute every field than to keep live across the loop
-    int k0 = k0c, g = gc, tk = tk0, tcl = tc;
-    unsigned vt = vtc, vs = vts;
+    int g = gc;
     int lt[M];
 #pragma unroll
     for (int i = 0; i < M; i++) lt[i] = ltc[i];
-    asm volatile("" : "+v"(k0), "+v"(g), "+v"(tk), "+v"(tcl), "+v"(vt), "+v"(vs));
+    asm volatile("" : "+v"(g));
 #pragma unroll
     for (int i = 0; i < M; i++) asm volatile("" : "+v"(lt[i]));
     auto rec_of = [&](int l) -> float4 & { return rw_lds[RC::at(wc, l - 1)]; };
+    const int k0 = k0c;
 
     const bool is_pt = f >= a.nq;
     const int iv = is_pt ? 1 : 0;
@@ -746,18 +759,18 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
     const bool lim = akord <= 16;
     // stage the field; then each lane takes its column's levels
 #pragma unroll
-    for (int j = 0; j < M; j++) s_in[ST::at(tcl, tk + j * KS)] = qn[j];
+    for (int j = 0; j < M; j++) s_in[sbt + j * KS + ((j * KS) >> 5)] = qn[j];
     __syncthreads();
     if (f + 1 < a.nf) {  // next field's elements, in flight while this one is computed
       const int fn = f + 1;
-      const Rsrc rn = make_rsrc(fn >= a.nq ? a.pt_src : a.q_src + (long long)fn * km * a.s2, fbytes);
+      const float *src = fn >= a.nq ? a.pt_src : a.q_src + (long long)fn * km * a.s2;
 #pragma unroll
-      for (int j = 0; j < M; j++) qn[j] = ldf(rn, vt + (unsigned)j * kstep, 0);
+      for (int j = 0; j < M; j++) qn[j] = ldf(make_rsrc(src, fbytes), vtc + (unsigned)j * kstep, 0);
     }
     const Rsrc rdst = make_rsrc(is_pt ? a.pt_dst : a.q_dst + (long long)f * km * a.s2, fbytes);
     float q[M];
 #pragma unroll
-    for (int i = 0; i < M; i++) q[i] = s_in[ST::at(wc, k0 + i)];
+    for (int i = 0; i < M; i++) q[i] = s_in[slr + i];
 
     // cell means of levels k0-1 .. k0+M+2 (A(j) is level k0+j+1)
     const float am2 = nb_lo<L>(q[M - 2]), am1 = nb_lo<L>(q[M - 1]);
@@ -819,10 +832,18 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
         const float up = fmaxf(x[i], lo);
         const float dn = fmaxf(fminf(x[i], hi), dn_floor);
         float v = glo * ghi > 0.f ? both : (glo > 0.f ? up : dn);
-        if (i == 0 && g == 0) v = x[i];
-        if (i == 1 && g == 0) v = both;
-        if (i == s_km && g == g_km) v = both;
-        if (i == s_kb && g == g_kb) v = x[i];
+        if (i == 0) {
+          if (g == 0) v = x[i];
+        }
+        if (i == 1) {
+          if (g == 0) v = both;
+        }
+        if (i == s_km) {
+          if (g == g_km) v = both;
+        }
+        if (i == s_kb) {
+          if (g == g_kb) v = x[i];
+        }
         x[i] = v;
       }
     }
@@ -838,7 +859,12 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
         const float a0 = A(i);
         bool em = (a0 - A(i - 1)) * (A(i + 1) - a0) < 0.f;
         // cells 1 and km: extremum against their own interface values
-        if ((i == 0 && g == 0) || (i == s_km && g == g_km)) em = (QL(i) - a0) * (QL(i + 1) - a0) > 0.f;
+        if (i == 0) {
+          if (g == 0) em = (QL(i) - a0) * (QL(i + 1) - a0) > 0.f;
+        }
+        if (i == s_km) {
+          if (g == g_km) em = (QL(i) - a0) * (QL(i + 1) - a0) > 0.f;
+        }
         unsigned e = em ? 1u : 0u;
         if (akord > 9) {
           const float x0 = 2.f * a0 - (QL(i) + QL(i + 1)), x1 = fabsf(QL(i) - QL(i + 1));
@@ -852,27 +878,41 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
 #pragma unroll
       for (int i = 0; i < M; i++) {
         float a1 = q[i], a2 = QL(i), a3 = QL(i + 1), a4;
-        // the interior form (3 stands for any cell of 3 .. km-2); cells 1, 2,
-        // km-1 and km in their own forms
-        int c = 3;
-        if (i == 1 && g == 0) c = 2;
-        if (i == s_km1 && g == g_km1) c = km - 1;
-        if (i == s_km && g == g_km) c = km;
-        if (i == 0 && g == 0) {
-          if (iv == 0) a2 = fmaxf(0.f, a2);
-          a4 = 3.f * (2.f * a1 - (a2 + a3));
-          cs_lim1<float>(a1, a2, a3, a4, (EF(0) & 1) != 0, 1);
-        } else {
-          cell_limit(c, km, iv, akord, qmin, a1, a2, a3, a4, EF(i - 1), EF(i), EF(i + 1), A(i - 1) - A(i - 2),
-                     A(i) - A(i - 1), A(i + 1) - A(i), A(i + 2) - A(i + 1));
+        cell_limit_interior(iv, akord, qmin, a1, a2, a3, a4, EF(i - 1), EF(i), EF(i + 1), A(i - 1) - A(i - 2),
+                            A(i) - A(i - 1), A(i + 1) - A(i), A(i + 2) - A(i + 1));
+        // cells 1, 2, km-1 and km take their own forms (at slots the whole
+        // wave knows, in the lanes that hold them)
+        // (lmt: cs_limiters mode; clip: for tracers the top (1) or bottom (2)
+        // interface value is bounded below by 0)
+        auto edge = [&](int lmt, int clip) {
+          float b1 = a1, b2 = QL(i), b3 = QL(i + 1);
+          if (iv == 0 && clip == 1) b2 = fmaxf(0.f, b2);
+          if (iv == 0 && clip == 2) b3 = fmaxf(0.f, b3);
+          float b4 = 3.f * (2.f * b1 - (b2 + b3));
+          cs_lim1<float>(b1, b2, b3, b4, (EF(i) & 1) != 0, lmt);
+          a2 = b2;
+          a3 = b3;
+          a4 = b4;
+        };
+        if (i == 0) {
+          if (g == 0) edge(1, 1);  // top cell
         }
-        rec_of(k0 + i + 1) = make_float4(a2, 0.5f * (a4 + a3 - a2), a4 * (1.0f / 3.0f), dp1[i] * q[i]);
+        if (i == 1) {
+          if (g == 0) edge(2, 0);
+        }
+        if (i == s_km1) {
+          if (g == g_km1) edge(2, 0);
+        }
+        if (i == s_km) {
+          if (g == g_km) edge(1, 2);  // bottom cell
+        }
+        rw_lds[srec + i] = make_float4(a2, 0.5f * (a4 + a3 - a2), a4 * (1.0f / 3.0f), dp1[i] * q[i]);
       }
     } else {
 #pragma unroll
       for (int i = 0; i < M; i++) {
         const float a2 = QL(i), a3 = QL(i + 1), a4 = 3.f * (2.f * q[i] - (a2 + a3));
-        rec_of(k0 + i + 1) = make_float4(a2, 0.5f * (a4 + a3 - a2), a4 * (1.0f / 3.0f), dp1[i] * q[i]);
+        rw_lds[srec + i] = make_float4(a2, 0.5f * (a4 + a3 - a2), a4 * (1.0f / 3.0f), dp1[i] * q[i]);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -892,11 +932,11 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
       const float top = pmean(ct.x, ct.y, ct.z, s_t, same ? s_b : 1.f);
       float acc = wb[i] * pmean(cb.x, cb.y, cb.z, 0.f, s_b);
       for (int l = l_b - 1; l > l_t; l--) acc += rec_of(l).w;
-      s_out[ST::at(wc, k0 + i)] = same ? top : (acc + wt[i] * top) * rd[i];
+      s_out[slr + i] = same ? top : (acc + wt[i] * top) * rd[i];
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < M; j++) stf(rdst, vs + (unsigned)j * kstep, s_out[ST::at(tcl, tk + j * KS)]);
+    for (int j = 0; j < M; j++) stf(rdst, vts + (unsigned)j * kstep, s_out[sbt + j * KS + ((j * KS) >> 5)]);
   }
 }
 
