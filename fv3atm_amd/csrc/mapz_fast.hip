@@ -715,6 +715,11 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
   const float stn_n = nb_hi<L>(st[0]);
   const int ltn = gc == L - 1 ? km : ltn_n;
   const float stn = gc == L - 1 ? 1.f : stn_n;
+  // LDS indices of the records of those source layers
+  int ot[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) ot[i] = RC::at(wc, ltc[i] - 1);
+  const int otn = RC::at(wc, ltn - 1);
   // slopes of the lane's forward (Q'(k0) -> Q'(k0+M)) and backward
   // (Q(k0+M+1) -> Q(k0+1)) maps, and their products at the scan steps
   float fs[5], bs[5];
@@ -743,12 +748,7 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
     // level-dependent offsets, masks and LDS addresses are cheaper to
     // recompute every field than to keep live across the loop
     int g = gc;
-    int lt[M];
-#pragma unroll
-    for (int i = 0; i < M; i++) lt[i] = ltc[i];
     asm volatile("" : "+v"(g));
-#pragma unroll
-    for (int i = 0; i < M; i++) asm volatile("" : "+v"(lt[i]));
     auto rec_of = [&](int l) -> float4 & { return rw_lds[RC::at(wc, l - 1)]; };
     const int k0 = k0c;
 
@@ -828,7 +828,7 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
         const float am = A(i - 1), a0 = A(i);
         const float lo = fminf(am, a0), hi = fmaxf(am, a0);
         const float glo = am - A(i - 2), ghi = A(i + 1) - a0;
-        const float both = fmaxf(fminf(x[i], hi), lo);
+        const float both = __builtin_amdgcn_fmed3f(x[i], lo, hi);  // lo <= hi: the clamp
         const float up = fmaxf(x[i], lo);
         const float dn = fmaxf(fminf(x[i], hi), dn_floor);
         float v = glo * ghi > 0.f ? both : (glo > 0.f ? up : dn);
@@ -922,9 +922,9 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
     // ---- Eulerian layers k0+1 .. k0+M
 #pragma unroll
     for (int i = 0; i < M; i++) {
-      const int l_t = lt[i], l_b = i + 1 < M ? lt[i + 1] : ltn;
+      const int l_t = ltc[i], l_b = i + 1 < M ? ltc[i + 1] : ltn;
       const float s_t = st[i], s_b = i + 1 < M ? st[i + 1] : stn;
-      const float4 ct = rec_of(l_t), cb = rec_of(l_b);
+      const float4 ct = rw_lds[ot[i]], cb = rw_lds[i + 1 < M ? ot[i + 1] : otn];
       // within one source layer: the parabola's mean over [s_t, s_b]; else
       // the bottom piece, the whole layers between, and the top piece over
       // [s_t, 1], weighted by their pressure widths
