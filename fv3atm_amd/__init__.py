@@ -522,7 +522,8 @@ DEFAULT_LAUNCH_CFG = 9
 def set_launch(ctx: Context, cfg: int = DEFAULT_LAUNCH_CFG, rows: int = 0, remap: int = 0):
     """Kernel choices of the FP32 step: transport launch shape (cfg, the
     library default 9; rows) -- results unchanged -- and remap kernel (0
-    column-shared, 1 per column)."""
+    column-shared with a column's levels across the lanes of a wave, 1 per
+    column, 2 column-shared with one thread per (column, field))."""
     check(_ffi.lib().fv3_ctx_set_launch(ctx.handle, cfg, rows, remap), "fv3_ctx_set_launch")
 
 
