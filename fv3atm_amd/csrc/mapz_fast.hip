@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "fv3_internal.h"
 #include "mapz_column.h"
@@ -623,7 +624,7 @@ struct Recs {
 template <int M, int L>
 size_t rows_lds_bytes() {
   constexpr int wgc = RW_WAVES * (64 / L);
-  return sizeof(float4) * wgc * Recs<L * M>::row + 2 * sizeof(float) * wgc * Stage<L * M>::row;
+  return sizeof(float4) * wgc * Recs<L * M>::row + 4 * sizeof(float) * wgc * Stage<L * M>::row;
 }
 
 template <int M, int L>
@@ -635,10 +636,13 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
   using ST = Stage<L * M>;
   using RC = Recs<L * M>;
   // LDS: cell records (c0 c1 c2 mass) of the workgroup's columns, then the
-  // field's input and output staging tiles
+  // input and output staging tiles of two consecutive fields (one barrier per
+  // field: field f's output tile is stored after field f+1's barrier)
   extern __shared__ float4 rw_lds[];
-  float *const s_in = reinterpret_cast<float *>(rw_lds + WGC * RC::row);
-  float *const s_out = s_in + WGC * ST::row;
+  float *const s_tiles = reinterpret_cast<float *>(rw_lds + WGC * RC::row);
+  auto tile_in = [&](int f) { return s_tiles + (f & 1) * 2 * WGC * ST::row; };
+  auto tile_out = [&](int f) { return s_tiles + ((f & 1) * 2 + 1) * WGC * ST::row; };
+  auto field_dst = [&](int f) { return f >= a.nq ? a.pt_dst : a.q_dst + (long long)f * a.km * a.s2; };
   const int tid = threadIdx.x, lane = tid & 63, cw = lane / L;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   // blocks are dealt round-robin to the 8 XCDs: renumber so that neighbouring
@@ -759,18 +763,22 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
     const bool lim = akord <= 16;
     // stage the field; then each lane takes its column's levels
 #pragma unroll
-    for (int j = 0; j < M; j++) s_in[sbt + j * KS + ((j * KS) >> 5)] = qn[j];
+    for (int j = 0; j < M; j++) tile_in(f)[sbt + j * KS + ((j * KS) >> 5)] = qn[j];
     __syncthreads();
+    if (f > 0) {  // the previous field's results, staged before the barrier
+      const Rsrc rprev = make_rsrc(field_dst(f - 1), fbytes);
+#pragma unroll
+      for (int j = 0; j < M; j++) stf(rprev, vts + (unsigned)j * kstep, tile_out(f - 1)[sbt + j * KS + ((j * KS) >> 5)]);
+    }
     if (f + 1 < a.nf) {  // next field's elements, in flight while this one is computed
       const int fn = f + 1;
       const float *src = fn >= a.nq ? a.pt_src : a.q_src + (long long)fn * km * a.s2;
 #pragma unroll
       for (int j = 0; j < M; j++) qn[j] = ldf(make_rsrc(src, fbytes), vtc + (unsigned)j * kstep, 0);
     }
-    const Rsrc rdst = make_rsrc(is_pt ? a.pt_dst : a.q_dst + (long long)f * km * a.s2, fbytes);
     float q[M];
 #pragma unroll
-    for (int i = 0; i < M; i++) q[i] = s_in[slr + i];
+    for (int i = 0; i < M; i++) q[i] = tile_in(f)[slr + i];
 
     // cell means of levels k0-1 .. k0+M+2 (A(j) is level k0+j+1)
     const float am2 = nb_lo<L>(q[M - 2]), am1 = nb_lo<L>(q[M - 1]);
@@ -932,11 +940,15 @@ __global__ void __launch_bounds__(64 * RW_WAVES, 4) k_remap_rows(const RowsArgs 
       const float top = pmean(ct.x, ct.y, ct.z, s_t, same ? s_b : 1.f);
       float acc = wb[i] * pmean(cb.x, cb.y, cb.z, 0.f, s_b);
       for (int l = l_b - 1; l > l_t; l--) acc += rec_of(l).w;
-      s_out[slr + i] = same ? top : (acc + wt[i] * top) * rd[i];
+      tile_out(f)[slr + i] = same ? top : (acc + wt[i] * top) * rd[i];
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  {
+    const int f = a.nf - 1;
+    const Rsrc rlast = make_rsrc(field_dst(f), fbytes);
 #pragma unroll
-    for (int j = 0; j < M; j++) stf(rdst, vts + (unsigned)j * kstep, s_out[sbt + j * KS + ((j * KS) >> 5)]);
+    for (int j = 0; j < M; j++) stf(rlast, vts + (unsigned)j * kstep, tile_out(f)[sbt + j * KS + ((j * KS) >> 5)]);
   }
 }
 
@@ -1052,6 +1064,18 @@ int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_d
     if (nwg > 0x7fffffffLL) return set_error("remap grid too large");
     ra.nwg = (unsigned)nwg;
     const size_t lds = M == 2 ? rows_lds_bytes<2, 16>() : L == 16 ? rows_lds_bytes<4, 16>() : rows_lds_bytes<4, 32>();
+    // more than 64 KB of dynamic LDS per workgroup must be allowed per kernel,
+    // once per process (outside any stream capture that follows)
+    {
+      static std::once_flag once[3];
+      static hipError_t err[3];
+      const int v = M == 2 ? 0 : L == 16 ? 1 : 2;
+      const void *fn = v == 0 ? reinterpret_cast<const void *>(&k_remap_rows<2, 16>)
+                       : v == 1 ? reinterpret_cast<const void *>(&k_remap_rows<4, 16>)
+                                : reinterpret_cast<const void *>(&k_remap_rows<4, 32>);
+      std::call_once(once[v], [&] { err[v] = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); });
+      FV3_HIP(err[v]);
+    }
     if (M == 2)
       hipLaunchKernelGGL((k_remap_rows<2, 16>), dim3((unsigned)nwg), dim3(64 * RW_WAVES), lds, s, ra);
     else if (L == 16)
