@@ -11,6 +11,7 @@
 #include "../../include/fv3atm_amd.h"
 #include "fused.h"
 #include "fv3_internal.h"
+#include "mapz_fast.h"
 #include "mapz_column.h"
 
 namespace fv3 {
@@ -185,7 +186,7 @@ int fv3_ctx_create(const fv3_bounds *bd, int npx, int npy, int npz, int ntiles, 
   c.npz = npz;
   c.ntiles = ntiles;
   (void)hipGetDevice(&c.dev);
-  if (noalloc_setup(c.dev)) {
+  if (noalloc_setup(c.dev) || remap_rows_setup()) {
     delete h;
     return 1;
   }

@@ -35,6 +35,8 @@ RemapCols remap_cols_carve(void *base, int km, long long ncol);
 // and, when pt_src is given, pt (iv=1, kord_tm, t_min) into pt_dst.  Sources
 // and destinations must not overlap.
 int remap_fast_check(int km, long long ncol, long long s2, int kord_tr, int kord_tm, bool with_pt);
+// once per process, before the first remap launch: the lane remap kernel's LDS limit
+int remap_rows_setup();
 int remap_fast_prep_f32(Ctx &c, int km, const double *ak, const double *bk, double ptop,
                         const double *dp1, double *delp_out, const RemapCols &cols, hipStream_t s);
 int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_dst, const float *pt_src,

@@ -1026,6 +1026,23 @@ int remap_fast_prep_f32(Ctx &c, int km, const double *ak, const double *bk, doub
   return 0;
 }
 
+// More than 64 KB of dynamic LDS per workgroup must be allowed per kernel:
+// done once per process, at context creation (before any stream capture).
+int remap_rows_setup() {
+  static std::once_flag once;
+  static hipError_t err = hipSuccess;
+  std::call_once(once, [] {
+    const void *fn[3] = {reinterpret_cast<const void *>(&k_remap_rows<2, 16>),
+                         reinterpret_cast<const void *>(&k_remap_rows<4, 16>),
+                         reinterpret_cast<const void *>(&k_remap_rows<4, 32>)};
+    const size_t lds[3] = {rows_lds_bytes<2, 16>(), rows_lds_bytes<4, 16>(), rows_lds_bytes<4, 32>()};
+    for (int v = 0; v < 3 && err == hipSuccess; v++)
+      err = hipFuncSetAttribute(fn[v], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds[v]);
+  });
+  FV3_HIP(err);
+  return 0;
+}
+
 int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_dst, const float *pt_src,
                           float *pt_dst, int kord_tr, int kord_tm, double t_min, const RemapCols &cols,
                           hipStream_t s) {
@@ -1064,18 +1081,7 @@ int remap_fast_fields_f32(Ctx &c, int km, int nq, const float *q_src, float *q_d
     if (nwg > 0x7fffffffLL) return set_error("remap grid too large");
     ra.nwg = (unsigned)nwg;
     const size_t lds = M == 2 ? rows_lds_bytes<2, 16>() : L == 16 ? rows_lds_bytes<4, 16>() : rows_lds_bytes<4, 32>();
-    // more than 64 KB of dynamic LDS per workgroup must be allowed per kernel,
-    // once per process (outside any stream capture that follows)
-    {
-      static std::once_flag once[3];
-      static hipError_t err[3];
-      const int v = M == 2 ? 0 : L == 16 ? 1 : 2;
-      const void *fn = v == 0 ? reinterpret_cast<const void *>(&k_remap_rows<2, 16>)
-                       : v == 1 ? reinterpret_cast<const void *>(&k_remap_rows<4, 16>)
-                                : reinterpret_cast<const void *>(&k_remap_rows<4, 32>);
-      std::call_once(once[v], [&] { err[v] = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); });
-      FV3_HIP(err[v]);
-    }
+    if (remap_rows_setup()) return 1;
     if (M == 2)
       hipLaunchKernelGGL((k_remap_rows<2, 16>), dim3((unsigned)nwg), dim3(64 * RW_WAVES), lds, s, ra);
     else if (L == 16)
