@@ -442,7 +442,7 @@ def run_next_rows(args, torch, fa, case, d):
     b = N * N * L * 4 * (2 + 2 * 5)
     out["dyn_core"] = dict(ms_per_substep=round(ms, 3), achieved_GBps=round(b / ms / 1e6, 1),
                            frac_hbm=round(b / (ms * 1e-3) / HBM_PEAK, 4),
-                           workload=f"C{N} L{L} FP32, {n_split} sub-steps, staged stage kernels")
+                           workload=f"C{N} L{L} FP32, {n_split} sub-steps, one launch per sub-step")
     log(f"dyn_core fluxes: {ms:.2f} ms per acoustic sub-step")
     del uc, vc, dev, d0
     if args.io:

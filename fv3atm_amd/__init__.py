@@ -342,7 +342,7 @@ def dyn_core_fluxes(ctx: Context, form, n_split, bdt, hord_dp, uc, vc, delp, dp1
     nid, njd = b.ied - b.isd + 1, b.jed - b.jsd + 1
     ni, nj = b.ie - b.is_ + 1, b.je - b.js + 1
     per = ctx.ntiles * ctx.npz
-    dt = torch.float64 if form == FORM_PLAIN64 else torch.float32
+    dt = torch.float32 if form == FORM_FUSED32 else torch.float64
     need = dict(uc=n_split * per * (ni + 1) * njd, vc=n_split * per * nid * (nj + 1), delp=per * nid * njd,
                 dp1=per * nid * njd, cx=per * (ni + 1) * njd, cy=per * nid * (nj + 1), mfx=per * (ni + 1) * nj,
                 mfy=per * ni * (nj + 1))

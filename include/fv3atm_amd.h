@@ -313,8 +313,11 @@ int fv3_kmp(int npz, double p_ref, const double *ak, const double *bk, int *kmp)
  * copy_corners acting on delp), adds crx, cry and the delp fluxes into cx,
  * cy, mfx, mfy, updates delp by the flux divergence and, on six faces,
  * refreshes delp's edge halos before the next sub-step (one tile: frozen).
- * Device arrays, tiles outermost; element type double (FV3_FORM_PLAIN64) or
- * float (FV3_FORM_FUSED32):
+ * FV3_FORM_PLAIN64 runs each sub-step as a sequence of stage kernels;
+ * FV3_FORM_FUSED64 and FV3_FORM_FUSED32 run it as one launch whose blocks keep
+ * the intermediates on chip (both FP64 forms equal the FP64 oracle bit for
+ * bit).  Device arrays, tiles outermost; element type double
+ * (FV3_FORM_PLAIN64, FV3_FORM_FUSED64) or float (FV3_FORM_FUSED32):
  *   uc: n_split sub-steps of (is:ie+1, jsd:jed, npz) per tile (sub-step
  *       outermost), vc: of (isd:ied, js:je+1, npz);
  *   delp (isd:ied, jsd:jed, npz): in, the thickness at the start with its

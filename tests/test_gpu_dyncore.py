@@ -1,8 +1,11 @@
 """dyn_core's accumulation of the tracer inputs (fv3_dyn_core_fluxes) on the
 GPU against the FP64 oracle (fv3o_dyn_core_fluxes): bit for bit in FP64 on
-one tile and on six faces (delp halos refreshed between sub-steps), within
-FP32 rounding in FP32; then end to end, the outputs drive tracer_2d_1L and
-a constant tracer stays constant."""
+one tile and on six faces (delp halos refreshed between sub-steps), in the
+staged form and in the one-launch-per-sub-step form (blocks of 32 x 8 cells:
+sizes 16 and 40 give one block, and full and partial blocks), which also
+equal each other element for element, halos included; within FP32 rounding in
+FP32; then end to end, the outputs drive tracer_2d_1L and a constant tracer
+stays constant."""
 import numpy as np
 import pytest
 import torch
@@ -22,7 +25,7 @@ def _dev(a, dt):
 
 
 def _run(case, form, n_split, uc, vc, bdt):
-    dt = np.float64 if form == fa.FORM_PLAIN64 else np.float32
+    dt = np.float32 if form == fa.FORM_FUSED32 else np.float64
     ctx = fa.Context(case.grids, case.npz)
     ref = oracle.dyn_core_fluxes(case, uc, vc, n_split, bdt)
     dev = dict(uc=_dev(uc, dt), vc=_dev(vc, dt), delp=_dev(case.dp1, dt))
@@ -41,13 +44,15 @@ def _interior(k, a):
     return a[NG:-NG, NG:-NG] if k in ("delp", "dp1") else a
 
 
-@pytest.mark.parametrize("nt,n_split", [(1, 6), (6, 3)])
-def test_fp64_matches_oracle(gpu_ready, nt, n_split):
-    case = cases.make_case(f"{nt}xC16L6", n=16, npz=6, nq=1, ntiles=nt, courant=1.8, delp_kind=1,
+@pytest.mark.parametrize("form,nt,n,n_split", [(fa.FORM_PLAIN64, 1, 16, 6), (fa.FORM_PLAIN64, 6, 16, 3),
+                                                (fa.FORM_FUSED64, 1, 16, 6), (fa.FORM_FUSED64, 6, 16, 3),
+                                                (fa.FORM_FUSED64, 1, 40, 5), (fa.FORM_FUSED64, 6, 40, 2)])
+def test_fp64_matches_oracle(gpu_ready, form, nt, n, n_split):
+    case = cases.make_case(f"{nt}xC{n}L6", n=n, npz=6, nq=1, ntiles=nt, courant=1.8, delp_kind=1,
                            delp_amp=(0.05, 0.2), seed=8)
-    print(f"[dyn_core] FP64 {nt} tile(s) n_split={n_split}", flush=True)
+    print(f"[dyn_core] FP64 form {form} {nt} tile(s) C{n} n_split={n_split}", flush=True)
     uc, vc = cases.c_grid_winds(case, n_split)
-    got, ref = _run(case, fa.FORM_PLAIN64, n_split, uc, vc, case.dt / n_split)
+    got, ref = _run(case, form, n_split, uc, vc, case.dt / n_split)
     for k in KEYS:
         a, b = _interior(k, got[k]), _interior(k, ref[k])
         err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
@@ -55,8 +60,11 @@ def test_fp64_matches_oracle(gpu_ready, nt, n_split):
         assert err <= 1e-12, k
 
 
-def test_fp32_error(gpu_ready):
-    case = cases.make_case("C32L8", n=32, npz=8, nq=1, courant=1.5, delp_kind=1, delp_amp=(0.05, 0.2), seed=2)
+@pytest.mark.parametrize("nt,n", [(1, 32), (1, 40), (6, 24)])
+def test_fp32_error(gpu_ready, nt, n):
+    case = cases.make_case(f"{nt}xC{n}L8", n=n, npz=8, nq=1, ntiles=nt, courant=1.5, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=2)
+    print(f"[dyn_core] FP32 {nt} tile(s) C{n}", flush=True)
     n_split = 4
     uc, vc = cases.c_grid_winds(case, n_split)
     got, ref = _run(case, fa.FORM_FUSED32, n_split, uc, vc, case.dt / n_split)
@@ -64,6 +72,20 @@ def test_fp32_error(gpu_ready):
         e = rel_l2(_interior(k, got[k]), _interior(k, ref[k]))
         print(f"  FP32 {k}: rel L2 {e:.2e}")
         assert e <= 1e-6, k
+
+
+@pytest.mark.parametrize("nt,n,n_split", [(1, 40, 3), (6, 24, 4)])
+def test_fused64_equals_staged(gpu_ready, nt, n, n_split):
+    # the one-launch form against the stage kernels, every element of every
+    # array (delp's halo and corner blocks included)
+    case = cases.make_case(f"{nt}xC{n}L4", n=n, npz=4, nq=1, ntiles=nt, courant=1.6, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=12)
+    print(f"[dyn_core] fused vs staged {nt} tile(s) C{n} n_split={n_split}", flush=True)
+    uc, vc = cases.c_grid_winds(case, n_split)
+    a, _ = _run(case, fa.FORM_PLAIN64, n_split, uc, vc, case.dt / n_split)
+    b, _ = _run(case, fa.FORM_FUSED64, n_split, uc, vc, case.dt / n_split)
+    for k in KEYS:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
 def test_constant_tracer_through_the_path(gpu_ready):
