@@ -129,6 +129,22 @@ def test_lane_remap_layouts(gpu_ready, npz, kord_tr, kord_tm):
     assert rel_l2(a, r) <= 1e-6
 
 
+def test_lane_remap_many_workgroups(gpu_ready):
+    # C96 L127: 9216 columns in 576 workgroups dealt over the XCDs; the lane
+    # kernel against the per-(column, field) kernel, and column mass
+    case = structured_case(courant=0.6, seed=13, n=96, npz=127, nq=6)
+    print("case", case.name, flush=True)
+    lanes = run32(case, 0, 9, 10)
+    per_field = run32(case, 2, 9, 10)
+    for iq in range(case.nq):
+        e = rel_l2(interior(lanes["q"][..., iq]), interior(per_field["q"][..., iq]))
+        print(f"tracer {iq}: rel L2 lanes-per_field {e:.2e}")
+        assert e <= 1e-6
+    e = rel_l2(interior(lanes["pt"]), interior(per_field["pt"]))
+    print(f"pt: rel L2 lanes-per_field {e:.2e}")
+    assert e <= 1e-6
+
+
 @pytest.mark.parametrize("kord", [9, 10, 22])
 def test_fast_remap_linear_in_pressure_is_exact(gpu_ready, kord):
     # no transport (Courant 1e-9): the remap alone acts on q = 2 + 3 p / ps,
