@@ -204,7 +204,12 @@ __device__ __forceinline__ void row_fetch(const FusedArgs<T> &a, const StripCtx<
 
 // One input row r = j0 - 3 + t of the streaming pass; U = t mod 6.  HO: the
 // advection order when fixed at compile time (0: taken from the arguments).
-template <typename T, int NT, int U, int HO, int SIDE = 0>
+// TW = t for the block's first six input rows, 6 after them.  Those rows
+// (r < j0 + 3) feed only the inner x-sweep, q_j and the two y-pipes: the
+// q_i, the outer x-sweep and the update of row r-3 < j0 are never used, and
+// the y-pipes' stages start as their inputs fill (first needed face: j0, at
+// t = 5).
+template <typename T, int NT, int U, int HO, int SIDE = 0, int TW = 6>
 __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T> &sc,
                                          LvState<T> &lv, TrState<T> (&st)[NT], const bool (&act)[NT],
                                          int t, const RowIn<T, NT> &in) {
@@ -221,9 +226,9 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   lv.area_r[U] = area_r;
   const T cyu = in.cyu;
   // x * (1/nsplit) with 1/1 = 1 exactly: unconditional scaling rounds the same
-  const T cys = cyu * sc.frac;
-  T yfx;
-  {
+  const T cys = TW >= 5 ? cyu * sc.frac : T(0);
+  T yfx = T(0);
+  if constexpr (TW >= 5) {
     const bool pos = cyu > T(0);
     if constexpr (fp64)
       yfx = pos ? cyu * in.ya * in.ye * in.yb : cyu * in.yc * in.ye * in.yd;
@@ -246,26 +251,26 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   const T xfxp = from_right(xfx);
   const T rax = area_r + xfx - xfxp;
   const T area_m = lv.area_r[M3];
-  const T ray = area_m + lv.yfx_prev - yfx;
-  T rrax = T(0), rray = T(0);
+  T rrax = T(0), rray = T(0), ray = T(1);
+  if constexpr (TW >= 6) ray = area_m + lv.yfx_prev - yfx;
   if constexpr (!fp64) {
     rrax = __builtin_amdgcn_rcpf(rax);
-    rray = __builtin_amdgcn_rcpf(ray);
+    if constexpr (TW >= 6) rray = __builtin_amdgcn_rcpf(ray);
   }
   const T cxm_u = lv.cx_r[M3];  // cx of row m (row r-3 was loaded 3 steps ago)
-  const T cxm = cxm_u * sc.frac;
-  const T mfys = in.mfy * sc.frac;
-  const T mfxs = in.mfx * sc.frac;
-  const T mfxp = from_right(mfxs);
+  const T cxm = TW >= 6 ? cxm_u * sc.frac : T(0);
+  const T mfys = TW >= 5 ? in.mfy * sc.frac : T(0);
+  const T mfxs = TW >= 6 ? in.mfx * sc.frac : T(0);
+  const T mfxp = TW >= 6 ? from_right(mfxs) : T(0);
   // edge strips: dxa of the rows (FP64), or the cells' edge face weights (FP32)
   T dxa_m = T(1), dxa_r = T(1);
   if (sc.edge_strip) {
     const T *g = fp64 ? a.dxa + sc.tg : a.gex + sc.tgf;
-    dxa_m = g[(long long)(x.mc - a.jsd) * nid + sc.lo_q];
+    if constexpr (TW >= 6) dxa_m = g[(long long)(x.mc - a.jsd) * nid + sc.lo_q];
     dxa_r = g[x.orow + sc.lo_q];
   }
   // cube-edge role of cell r-2 along the columns (uniform across the wave)
-  const int role_y = sc.edge_rows ? edge_role(r - 2, a.js, a.je, a.npy, a.edge) : 0;
+  const int role_y = TW >= 4 && sc.edge_rows ? edge_role(r - 2, a.js, a.je, a.npy, a.edge) : 0;
   T dd0 = T(0), dd1 = T(0), dd2 = T(0), dd3 = T(0), dd4 = T(0);
   if (role_y != 0) {
     auto dya_at = [&](int jj) {
@@ -280,7 +285,7 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   }
   // output row: thickness and area factors shared by the tracers
   T rarea = T(0), d1 = T(0), rdp2 = T(0), dp2 = T(0);
-  if (x.out_row) {
+  if (TW >= 6 && x.out_row) {
     rarea = in.rarea;
     const T ddiv = ((mfxs - mfxp) + (lv.mfy_prev - mfys)) * rarea;
     T xd = (T)in.dp1;
@@ -297,15 +302,18 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     TrState<T> &s = st[n];
     const T qy = in.qy[n], qx = in.qx[n];
     // inner y-sweep at face r-2 and q_i of row r-3
-    const T fy2 = ypipe_step<T, U>(s.py, qy, cys, ord_in, role_y, dd0, dd1, dd2, dd3, dd4);
+    const T fy2 = ypipe_step<T, U, TW>(s.py, qy, cys, ord_in, role_y, dd0, dd1, dd2, dd3, dd4);
     const T qm = s.py.w[M3];
-    T q_i;
-    if constexpr (fp64)
-      q_i = (qm * area_m + lv.yfx_prev * s.fy2_prev - yfx * fy2) / ray;
-    else
-      q_i = (qm * area_m + lv.yfx_prev * s.fy2_prev - yfx * fy2) * rray;
     // outer x-sweep of row r-3, inner x-sweep of row r, q_j of row r
-    const T fx = xsweep<SIDE>(q_i, cxm, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, hord, dxa_m, sc.ec);
+    T fx = T(0);
+    if constexpr (TW >= 6) {
+      T q_i;
+      if constexpr (fp64)
+        q_i = (qm * area_m + lv.yfx_prev * s.fy2_prev - yfx * fy2) / ray;
+      else
+        q_i = (qm * area_m + lv.yfx_prev * s.fy2_prev - yfx * fy2) * rray;
+      fx = xsweep<SIDE>(q_i, cxm, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, hord, dxa_m, sc.ec);
+    }
     const T fx2 = xsweep<SIDE>(qx, cxr, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, ord_in, dxa_r, sc.ec);
     s.fx2_r[U] = fx2;
     const T fx2p = from_right(fx2);
@@ -315,8 +323,14 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     else
       q_j = (qx * area_r + xfx * fx2 - xfxp * fx2p) * rrax;
     // outer y-sweep at face r-2, fluxes, update of row r-3
-    const T fy = ypipe_step<T, U>(s.pj, q_j, cys, hord, role_y, dd0, dd1, dd2, dd3, dd4);
+    const T fy = ypipe_step<T, U, TW>(s.pj, q_j, cys, hord, role_y, dd0, dd1, dd2, dd3, dd4);
+    if constexpr (TW < 5) continue;
     const T Fy = T(0.5) * (fy + fy2) * mfys;
+    s.fy2_prev = fy2;
+    if constexpr (TW < 6) {
+      s.Fy_prev = Fy;
+      continue;
+    }
     const T Fx = T(0.5) * (fx + s.fx2_r[M3]) * mfxs;
     const T Fxp = from_right(Fx);
     if (x.out_row && sc.col_out) {
@@ -326,7 +340,6 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
       else
         orow[sc.lo_q] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) * rdp2;
     }
-    s.fy2_prev = fy2;
     s.Fy_prev = Fy;
   }
   lv.yfx_prev = yfx;
@@ -655,8 +668,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
         }
       };
       for (int t = 0; t < LR_DEPTH - 1 && t < nrow; t++) issue(t);
-      for (int t0 = 0; t0 < nrow; t0 += 6) {
-#define FV3_RROW(U)                                                                  \
+#define FV3_RROW(U, TW)                                                              \
   if (t0 + U < nrow) {                                                               \
     const int t = t0 + U;                                                            \
     ring_wait<W, NT, QN>(min(LR_DEPTH - 2, nrow - 1 - t));                           \
@@ -664,12 +676,20 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
     if (any) {                                                                       \
       RowIn<T, NT> in;                                                               \
       ring_read<W, NT, QN>(ring, wave, lane, t, in);                                 \
-      row_step<T, NT, U, HO, ek_side(EK)>(a, sc, lv, st, act, t, in); \
+      row_step<T, NT, U, HO, ek_side(EK), TW>(a, sc, lv, st, act, t, in);            \
     }                                                                                \
   }
-        FV3_RROW(0) FV3_RROW(1) FV3_RROW(2) FV3_RROW(3) FV3_RROW(4) FV3_RROW(5)
-#undef FV3_RROW
+      // the lean kind runs the block's first six input rows (the y-pipes'
+      // warm-up) as their own unrolled group; the edge kinds, held to 168
+      // VGPRs by their occupancy, would spill with it
+      if constexpr (EK == 0) {
+        constexpr int t0 = 0;
+        FV3_RROW(0, 0) FV3_RROW(1, 1) FV3_RROW(2, 2) FV3_RROW(3, 3) FV3_RROW(4, 4) FV3_RROW(5, 5)
       }
+      for (int t0 = EK == 0 ? 6 : 0; t0 < nrow; t0 += 6) {
+        FV3_RROW(0, 6) FV3_RROW(1, 6) FV3_RROW(2, 6) FV3_RROW(3, 6) FV3_RROW(4, 6) FV3_RROW(5, 6)
+      }
+#undef FV3_RROW
       continue;
     }
     // rows are fetched one ahead; the bare s_barrier keeps the waves of the

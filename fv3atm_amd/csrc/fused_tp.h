@@ -427,24 +427,34 @@ struct YPipe {
 
 // One step at input row r; U is the slot of row r.  role/dd: the cube-edge
 // role of cell r-2 and the spacings of cells r-4..r (uniform over the wave).
-template <typename T, int U>
+// TW: the row's index t when it is one of a block's first six input rows (the
+// window then holds rows 0..t only, and each stage runs from the first row at
+// which its inputs exist: dm from t = 2, face values from 3, cell
+// coefficients from 4, the flux from 5 -- the first face the block needs);
+// 6 in the steady state.  The flux is 0 before t = 5.
+template <typename T, int U, int TW = 6>
 __device__ __forceinline__ T ypipe_step(YPipe<T> &p, T qnew, T c, int iord, int role, T d0, T d1,
                                         T d2, T d3, T d4) {
   constexpr int s0 = U % 6, s1 = (U + 5) % 6, s2 = (U + 4) % 6, s3 = (U + 3) % 6,
                 s4 = (U + 2) % 6;  // rows r, r-1, r-2, r-3, r-4
   p.w[s0] = qnew;
-  p.dm[s1] = dm3(p.w[s2], p.w[s1], p.w[s0]);
-  p.al[s1] = al2(p.w[s2], p.w[s1], p.dm[s2], p.dm[s1]);
-  cell_coef(p.w[s4], p.w[s3], p.w[s2], p.w[s1], p.w[s0], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2],
-            p.al[s1], iord, p.bl[s2], p.br[s2]);
-  if (role != 0) {  // uniform over the wave
-    const T efl = edge_face(p.w[s4], p.w[s3], p.w[s2], p.w[s1], d0, d1, d2, d3);
-    const T efr = edge_face(p.w[s3], p.w[s2], p.w[s1], p.w[s0], d1, d2, d3, d4);
-    edge_coef_sel(p.w[s3], p.w[s2], p.w[s1], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2], p.al[s1], role, efl,
-                  efr, p.bl[s2], p.br[s2]);
+  if constexpr (TW >= 2) p.dm[s1] = dm3(p.w[s2], p.w[s1], p.w[s0]);
+  if constexpr (TW >= 3) p.al[s1] = al2(p.w[s2], p.w[s1], p.dm[s2], p.dm[s1]);
+  if constexpr (TW >= 4) {
+    cell_coef(p.w[s4], p.w[s3], p.w[s2], p.w[s1], p.w[s0], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2],
+              p.al[s1], iord, p.bl[s2], p.br[s2]);
+    if (role != 0) {  // uniform over the wave
+      const T efl = edge_face(p.w[s4], p.w[s3], p.w[s2], p.w[s1], d0, d1, d2, d3);
+      const T efr = edge_face(p.w[s3], p.w[s2], p.w[s1], p.w[s0], d1, d2, d3, d4);
+      edge_coef_sel(p.w[s3], p.w[s2], p.w[s1], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2], p.al[s1], role, efl,
+                    efr, p.bl[s2], p.br[s2]);
+    }
+    p.bs[s2] = p.bl[s2] + p.br[s2];
   }
-  p.bs[s2] = p.bl[s2] + p.br[s2];
-  return face_flux_s(c, p.w[s3], p.br[s3], p.bs[s3], p.w[s2], p.bl[s2], p.bs[s2]);
+  if constexpr (TW >= 5)
+    return face_flux_s(c, p.w[s3], p.br[s3], p.bs[s3], p.w[s2], p.bl[s2], p.bs[s2]);
+  else
+    return T(0);
 }
 
 }  // namespace fv3
