@@ -612,7 +612,7 @@ def main():
             log(f"launch cfg {cfg} rows {rows or 'default'}: transport {t:.2f} ms")
             if best is None or t < best[0]:
                 best = (t, cfg, rows)
-        for r in (42, 48, 60, 72, 90):
+        for r in (42, 48, 60, 72, 76, 90, 95, 127):
             if r == rows or r > N:
                 continue
             fa.set_launch(ctx, best[1], r, args.remap)

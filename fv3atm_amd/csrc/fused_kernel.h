@@ -55,7 +55,11 @@ struct RowIn {
   T ya, yb, yc, yd, ye;  // FP32: gyp gym; FP64: dya(j-1) sg3(j-1) dya(j) sg1(j) dx
   T xa, xb, xc, xd, xe;  // FP32: gxp gxm; FP64: dxa(i-1) sg2(i-1) dxa(i) sg0(i) dy
   double dp1;
+  T area_m, cxm_u;  // FP32: area and cx of row r-3, from the LDS ring
   T qy[NT], qx[NT];
+  T qw[NT][4];      // FP32: the y-sweep sources of rows r-1 .. r-4, from the LDS ring
+  T fx2m[NT];       // FP32: the inner x-flux of row r-3 (kept in the ring by this wave)
+  float *fx2w[NT];  // FP32: where this row's inner x-flux goes
 };
 
 // q sources per input row of a kind: the strips with corner cells read the
@@ -223,7 +227,7 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
 
   // ---- level data of this row (shared by the NT tracers)
   const T area_r = in.area;
-  lv.area_r[U] = area_r;
+  if constexpr (fp64) lv.area_r[U] = area_r;
   const T cyu = in.cyu;
   // x * (1/nsplit) with 1/1 = 1 exactly: unconditional scaling rounds the same
   const T cys = TW >= 5 ? cyu * sc.frac : T(0);
@@ -237,7 +241,7 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     yfx = yfx * sc.frac;
   }
   const T cxr_u = in.cxu;
-  lv.cx_r[U] = cxr_u;
+  if constexpr (fp64) lv.cx_r[U] = cxr_u;
   const T cxr = cxr_u * sc.frac;
   T xfx;
   {
@@ -250,14 +254,14 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   }
   const T xfxp = from_right(xfx);
   const T rax = area_r + xfx - xfxp;
-  const T area_m = lv.area_r[M3];
+  const T area_m = fp64 ? lv.area_r[M3] : in.area_m;
   T rrax = T(0), rray = T(0), ray = T(1);
   if constexpr (TW >= 6) ray = area_m + lv.yfx_prev - yfx;
   if constexpr (!fp64) {
     rrax = __builtin_amdgcn_rcpf(rax);
     if constexpr (TW >= 6) rray = __builtin_amdgcn_rcpf(ray);
   }
-  const T cxm_u = lv.cx_r[M3];  // cx of row m (row r-3 was loaded 3 steps ago)
+  const T cxm_u = fp64 ? lv.cx_r[M3] : in.cxm_u;  // cx of row m (row r-3, loaded 3 steps ago)
   const T cxm = TW >= 6 ? cxm_u * sc.frac : T(0);
   const T mfys = TW >= 5 ? in.mfy * sc.frac : T(0);
   const T mfxs = TW >= 6 ? in.mfx * sc.frac : T(0);
@@ -302,8 +306,8 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     TrState<T> &s = st[n];
     const T qy = in.qy[n], qx = in.qx[n];
     // inner y-sweep at face r-2 and q_i of row r-3
-    const T fy2 = ypipe_step<T, U, TW>(s.py, qy, cys, ord_in, role_y, dd0, dd1, dd2, dd3, dd4);
-    const T qm = s.py.w[M3];
+    const T fy2 = ypipe_step<T, U, TW, !fp64>(s.py, qy, in.qw[n], cys, ord_in, role_y, dd0, dd1, dd2, dd3, dd4);
+    const T qm = fp64 ? s.py.w[M3] : in.qw[n][2];
     // outer x-sweep of row r-3, inner x-sweep of row r, q_j of row r
     T fx = T(0);
     if constexpr (TW >= 6) {
@@ -315,7 +319,10 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
       fx = xsweep<SIDE>(q_i, cxm, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, hord, dxa_m, sc.ec);
     }
     const T fx2 = xsweep<SIDE>(qx, cxr, sc.i, a.is, a.ie, a.npx, a.edge, sc.edge_strip, ord_in, dxa_r, sc.ec);
-    s.fx2_r[U] = fx2;
+    if constexpr (fp64)
+      s.fx2_r[U] = fx2;
+    else
+      in.fx2w[n][0] = fx2;
     const T fx2p = from_right(fx2);
     T q_j;
     if constexpr (fp64)
@@ -323,7 +330,7 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     else
       q_j = (qx * area_r + xfx * fx2 - xfxp * fx2p) * rrax;
     // outer y-sweep at face r-2, fluxes, update of row r-3
-    const T fy = ypipe_step<T, U, TW>(s.pj, q_j, cys, hord, role_y, dd0, dd1, dd2, dd3, dd4);
+    const T fy = ypipe_step<T, U, TW>(s.pj, q_j, nullptr, cys, hord, role_y, dd0, dd1, dd2, dd3, dd4);
     if constexpr (TW < 5) continue;
     const T Fy = T(0.5) * (fy + fy2) * mfys;
     s.fy2_prev = fy2;
@@ -331,7 +338,7 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
       s.Fy_prev = Fy;
       continue;
     }
-    const T Fx = T(0.5) * (fx + s.fx2_r[M3]) * mfxs;
+    const T Fx = T(0.5) * (fx + (fp64 ? s.fx2_r[M3] : in.fx2m[n])) * mfxs;
     const T Fxp = from_right(Fx);
     if (x.out_row && sc.col_out) {
       T *orow = s.qout + x.corow;
@@ -346,17 +353,26 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   lv.mfy_prev = mfys;
 }
 
-// ---- LDS row ring of the lean FP32 kernel.  Rows are loaded straight into
-// LDS with global_load_lds_dword (the data in flight holds no VGPRs), so the
-// loads of a row can be issued LR_DEPTH-1 rows ahead.  The waves of a
+// ---- LDS row ring of the FP32 kernels.  Rows are loaded straight into LDS
+// with global_load_lds_dword (the data in flight holds no VGPRs), so the
+// loads of a row can be issued LR_AHEAD rows ahead.  The waves of a
 // workgroup split the LR_NL level arrays of a row between them (they stream
-// the same strip and level) and each loads its own tracers' rows.
-constexpr int LR_DEPTH = 4;
+// the same strip and level) and each loads its own tracers' rows.  The ring
+// also keeps the four rows behind the one computed: the stencil's values of
+// rows r-1 .. r-4 (y-sweep sources, area and cx of row r-3) are read back
+// from LDS instead of being carried in registers, and each wave keeps its
+// inner x-fluxes there for the three rows until they are averaged.  The
+// interior kernel then fits 168 VGPRs, three workgroups per CU.
+constexpr int LR_DEPTH = 8;  // slots: rows t-4 .. t+3 at row t
+constexpr int LR_AHEAD = 3;  // rows issued ahead of the row computed
 constexpr int LR_NL = 12;  // area cyu cxu mfy mfx rarea gyp gym gxp gxm dp1.lo dp1.hi
 
 template <int W, int NT, int QN>
 struct LdsRing {
+  // per slot: the level arrays and q sources loaded by the DMA
   float v[LR_DEPTH][LR_NL + W * NT * QN][FT_LANES];
+  // each wave's inner x-fluxes of rows t-3 .. t (written and read by the wave itself)
+  float fx2[4][W * NT][FT_LANES];
 };
 
 __device__ __forceinline__ void lds_dma(const float *src, float *dst_row) {
@@ -371,7 +387,7 @@ __device__ __forceinline__ void wait_vm_lgkm0() {
   __builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | (0x7 << 4));
 }
 
-// issue the loads of input row t into ring slot t % LR_DEPTH
+// issue the loads of input row t into ring slot t mod LR_DEPTH
 template <int W, int NT, int EK>
 __device__ __forceinline__ void ring_issue(const FusedArgs<float> &a, const StripCtx<float> &sc,
                                            const TrState<float> (&st)[NT],
@@ -379,7 +395,7 @@ __device__ __forceinline__ void ring_issue(const FusedArgs<float> &a, const Stri
   constexpr int QN = ek_qn(EK);
   const RowIdx<float> x = row_idx(a, sc, t);
   const int nid = a.nid;
-  float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
+  float(*slot)[FT_LANES] = ring.v[t & (LR_DEPTH - 1)];
   const long long frow = (long long)(x.fc - a.jsd) * nid;
   const long long gxrow = (long long)(x.rc - a.jsd) * (nid + 1) + (a.is - a.isd);
   // uniform row base (scalar) + the lane's column offset in that array's
@@ -464,7 +480,7 @@ template <int W, int NT>
 __device__ __forceinline__ void ring_issue_lean(const FusedArgs<float> &a, const StripCtx<float> &sc,
                                                 const TrState<float> (&st)[NT], const LeanRows<W> &lr,
                                                 LdsRing<W, NT, 1> &ring, int wave, int t) {
-  float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
+  float(*slot)[FT_LANES] = ring.v[t & (LR_DEPTH - 1)];
 #pragma unroll
   for (int m = 0; m < LeanRows<W>::M; m++)
     if (wave + m * W < LR_NL) lds_dma(lr.p[m] + (long long)t * lr.step[m] + lr.loff[m], slot[wave + m * W]);
@@ -479,7 +495,7 @@ __device__ __forceinline__ void ring_issue_mixed(const FusedArgs<float> &a, cons
                                                  const TrState<float> (&st)[NT], const LeanRows<W> &lr,
                                                  LdsRing<W, NT, ek_qn(EK)> &ring, int wave, int t) {
   constexpr int QN = ek_qn(EK);
-  float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
+  float(*slot)[FT_LANES] = ring.v[t & (LR_DEPTH - 1)];
 #pragma unroll
   for (int m = 0; m < LeanRows<W>::M; m++)
     if (wave + m * W < LR_NL) lds_dma(lr.p[m] + (long long)t * lr.step[m] + lr.loff[m], slot[wave + m * W]);
@@ -496,7 +512,10 @@ __device__ __forceinline__ void ring_issue_mixed(const FusedArgs<float> &a, cons
 template <int W, int NT, int QN>
 __device__ __forceinline__ void ring_read(const LdsRing<W, NT, QN> &ring, int wave, int lane, int t,
                                           RowIn<float, NT> &in) {
-  const float(*slot)[FT_LANES] = ring.v[t % LR_DEPTH];
+  const float(*slot)[FT_LANES] = ring.v[t & (LR_DEPTH - 1)];
+  const float(*slot_m)[FT_LANES] = ring.v[(t - 3) & (LR_DEPTH - 1)];
+  in.area_m = slot_m[0][lane];
+  in.cxm_u = slot_m[2][lane];
   in.area = slot[0][lane];
   in.cyu = slot[1][lane];
   in.cxu = slot[2][lane];
@@ -511,6 +530,10 @@ __device__ __forceinline__ void ring_read(const LdsRing<W, NT, QN> &ring, int wa
   in.dp1 = __longlong_as_double((long long)(lo | (hi << 32)));
 #pragma unroll
   for (int n = 0; n < NT; n++) {
+#pragma unroll
+    for (int d = 1; d <= 4; d++) in.qw[n][d - 1] = ring.v[(t - d) & (LR_DEPTH - 1)][LR_NL + (wave * NT + n) * QN][lane];
+    in.fx2m[n] = ring.fx2[(t - 3) & 3][wave * NT + n][lane];
+    in.fx2w[n] = const_cast<float *>(&ring.fx2[t & 3][wave * NT + n][lane]);
     in.qy[n] = slot[LR_NL + (wave * NT + n) * QN][lane];
     in.qx[n] = QN == 2 ? slot[LR_NL + (wave * NT + n) * QN + 1][lane] : in.qy[n];
   }
@@ -522,7 +545,7 @@ __device__ __forceinline__ void ring_read(const LdsRing<W, NT, QN> &ring, int wa
 template <int W, int NT, int QN>
 __device__ __forceinline__ void ring_wait(int ahead) {
   constexpr int NMIN = LR_NL / W + NT * QN;
-  static_assert((LR_DEPTH - 2) * NMIN < 64, "vmcnt range");
+  static_assert((LR_AHEAD - 1) * NMIN < 64, "vmcnt range");
   if (ahead >= 2)
     wait_vm_lgkm0<2 * NMIN>();
   else if (ahead == 1)
@@ -646,7 +669,7 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
     const bool any = act[0];
     if constexpr (sizeof(T) == 4) {
       constexpr int QN = ek_qn(EK);
-      static_assert(LR_DEPTH == 4, "ring_wait covers up to two rows ahead");
+      static_assert(LR_AHEAD == 3 && LR_DEPTH >= LR_AHEAD + 5, "ring_wait covers up to two rows ahead; rows t-4..t+3 kept");
       __shared__ LdsRing<W, NT, QN> ring;
       // the previous round's last reads of every slot are done before the
       // prologue overwrites them
@@ -667,12 +690,12 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
             ring_issue<W, NT, EK>(a, sc, st, ring, wave, t);
         }
       };
-      for (int t = 0; t < LR_DEPTH - 1 && t < nrow; t++) issue(t);
+      for (int t = 0; t < LR_AHEAD && t < nrow; t++) issue(t);
 #define FV3_RROW(U, TW)                                                              \
   if (t0 + U < nrow) {                                                               \
     const int t = t0 + U;                                                            \
-    ring_wait<W, NT, QN>(min(LR_DEPTH - 2, nrow - 1 - t));                           \
-    if (t + LR_DEPTH - 1 < nrow) issue(t + LR_DEPTH - 1);                            \
+    ring_wait<W, NT, QN>(min(LR_AHEAD - 1, nrow - 1 - t));                           \
+    if (t + LR_AHEAD < nrow) issue(t + LR_AHEAD);                                    \
     if (any) {                                                                       \
       RowIn<T, NT> in;                                                               \
       ring_read<W, NT, QN>(ring, wave, lane, t, in);                                 \

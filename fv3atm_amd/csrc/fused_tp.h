@@ -432,27 +432,30 @@ struct YPipe {
 // which its inputs exist: dm from t = 2, face values from 3, cell
 // coefficients from 4, the flux from 5 -- the first face the block needs);
 // 6 in the steady state.  The flux is 0 before t = 5.
-template <typename T, int U, int TW = 6>
-__device__ __forceinline__ T ypipe_step(YPipe<T> &p, T qnew, T c, int iord, int role, T d0, T d1,
+template <typename T, int U, int TW = 6, bool EXT = false>
+__device__ __forceinline__ T ypipe_step(YPipe<T> &p, T qnew, const T *wx, T c, int iord, int role, T d0, T d1,
                                         T d2, T d3, T d4) {
   constexpr int s0 = U % 6, s1 = (U + 5) % 6, s2 = (U + 4) % 6, s3 = (U + 3) % 6,
                 s4 = (U + 2) % 6;  // rows r, r-1, r-2, r-3, r-4
-  p.w[s0] = qnew;
-  if constexpr (TW >= 2) p.dm[s1] = dm3(p.w[s2], p.w[s1], p.w[s0]);
-  if constexpr (TW >= 3) p.al[s1] = al2(p.w[s2], p.w[s1], p.dm[s2], p.dm[s1]);
+  // the window's cells: held in the pipe, or (EXT) given as rows r-1 .. r-4
+  const T w0 = qnew;
+  if constexpr (!EXT) p.w[s0] = qnew;
+  const T w1 = EXT ? wx[0] : p.w[s1], w2 = EXT ? wx[1] : p.w[s2], w3 = EXT ? wx[2] : p.w[s3],
+          w4 = EXT ? wx[3] : p.w[s4];
+  if constexpr (TW >= 2) p.dm[s1] = dm3(w2, w1, w0);
+  if constexpr (TW >= 3) p.al[s1] = al2(w2, w1, p.dm[s2], p.dm[s1]);
   if constexpr (TW >= 4) {
-    cell_coef(p.w[s4], p.w[s3], p.w[s2], p.w[s1], p.w[s0], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2],
-              p.al[s1], iord, p.bl[s2], p.br[s2]);
+    cell_coef(w4, w3, w2, w1, w0, p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2], p.al[s1], iord, p.bl[s2], p.br[s2]);
     if (role != 0) {  // uniform over the wave
-      const T efl = edge_face(p.w[s4], p.w[s3], p.w[s2], p.w[s1], d0, d1, d2, d3);
-      const T efr = edge_face(p.w[s3], p.w[s2], p.w[s1], p.w[s0], d1, d2, d3, d4);
-      edge_coef_sel(p.w[s3], p.w[s2], p.w[s1], p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2], p.al[s1], role, efl,
-                    efr, p.bl[s2], p.br[s2]);
+      const T efl = edge_face(w4, w3, w2, w1, d0, d1, d2, d3);
+      const T efr = edge_face(w3, w2, w1, w0, d1, d2, d3, d4);
+      edge_coef_sel(w3, w2, w1, p.dm[s3], p.dm[s2], p.dm[s1], p.al[s2], p.al[s1], role, efl, efr, p.bl[s2],
+                    p.br[s2]);
     }
     p.bs[s2] = p.bl[s2] + p.br[s2];
   }
   if constexpr (TW >= 5)
-    return face_flux_s(c, p.w[s3], p.br[s3], p.bs[s3], p.w[s2], p.bl[s2], p.bs[s2]);
+    return face_flux_s(c, w3, p.br[s3], p.bs[s3], w2, p.bl[s2], p.bs[s2]);
   else
     return T(0);
 }
