@@ -510,7 +510,7 @@ __device__ __forceinline__ void ring_issue_mixed(const FusedArgs<float> &a, cons
 }
 
 template <int W, int NT, int QN>
-__device__ __forceinline__ void ring_read(const LdsRing<W, NT, QN> &ring, int wave, int lane, int t,
+__device__ __forceinline__ void ring_read(LdsRing<W, NT, QN> &ring, int wave, int lane, int t,
                                           RowIn<float, NT> &in) {
   const float(*slot)[FT_LANES] = ring.v[t & (LR_DEPTH - 1)];
   const float(*slot_m)[FT_LANES] = ring.v[(t - 3) & (LR_DEPTH - 1)];
@@ -533,7 +533,7 @@ __device__ __forceinline__ void ring_read(const LdsRing<W, NT, QN> &ring, int wa
 #pragma unroll
     for (int d = 1; d <= 4; d++) in.qw[n][d - 1] = ring.v[(t - d) & (LR_DEPTH - 1)][LR_NL + (wave * NT + n) * QN][lane];
     in.fx2m[n] = ring.fx2[(t - 3) & 3][wave * NT + n][lane];
-    in.fx2w[n] = const_cast<float *>(&ring.fx2[t & 3][wave * NT + n][lane]);
+    in.fx2w[n] = &ring.fx2[t & 3][wave * NT + n][lane];
     in.qy[n] = slot[LR_NL + (wave * NT + n) * QN][lane];
     in.qx[n] = QN == 2 ? slot[LR_NL + (wave * NT + n) * QN + 1][lane] : in.qy[n];
   }
