@@ -516,12 +516,12 @@ def set_pipeline(ctx: Context, tracers_per_chunk: int = 0):
     check(_ffi.lib().fv3_ctx_set_pipeline(ctx.handle, int(tracers_per_chunk)), "fv3_ctx_set_pipeline")
 
 
-DEFAULT_LAUNCH_CFG = 11
+DEFAULT_LAUNCH_CFG = 12
 
 
 def set_launch(ctx: Context, cfg: int = DEFAULT_LAUNCH_CFG, rows: int = 0, remap: int = 0):
     """Kernel choices of the FP32 step: transport launch shape (cfg, the
-    library default 11; rows) -- results unchanged -- and remap kernel (0
+    library default 12; rows) -- results unchanged -- and remap kernel (0
     column-shared with a column's levels across the lanes of a wave, 1 per
     column, 2 column-shared with one thread per (column, field))."""
     check(_ffi.lib().fv3_ctx_set_launch(ctx.handle, cfg, rows, remap), "fv3_ctx_set_launch")

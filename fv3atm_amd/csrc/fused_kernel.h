@@ -288,8 +288,10 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     dd4 = dya_at(r);
   }
   // output row: thickness and area factors shared by the tracers
+  // (FP32 reads rarea and dp1 of every row from the ring and computes them
+  // without a branch; only output rows store)
   T rarea = T(0), d1 = T(0), rdp2 = T(0), dp2 = T(0);
-  if (TW >= 6 && x.out_row) {
+  if (TW >= 6 && (!fp64 || x.out_row)) {
     rarea = in.rarea;
     const T ddiv = ((mfxs - mfxp) + (lv.mfy_prev - mfys)) * rarea;
     T xd = (T)in.dp1;
@@ -302,7 +304,9 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
   // ---- the tracers
 #pragma unroll
   for (int n = 0; n < NT; n++) {
-    if (!act[n]) continue;
+    // FP32: an inactive tracer of an active wave computes on tracer 0's rows
+    // (what its ring slots hold) and stores nothing, so no branch per tracer
+    if (fp64 && !act[n]) continue;
     TrState<T> &s = st[n];
     const T qy = in.qy[n], qx = in.qx[n];
     // inner y-sweep at face r-2 and q_i of row r-3
@@ -340,7 +344,7 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     }
     const T Fx = T(0.5) * (fx + (fp64 ? s.fx2_r[M3] : in.fx2m[n])) * mfxs;
     const T Fxp = from_right(Fx);
-    if (x.out_row && sc.col_out) {
+    if (act[n] && x.out_row && sc.col_out) {
       T *orow = s.qout + x.corow;
       if constexpr (fp64)
         orow[sc.lo_q] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) / dp2;
@@ -361,8 +365,8 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
 // also keeps the four rows behind the one computed: the stencil's values of
 // rows r-1 .. r-4 (y-sweep sources, area and cx of row r-3) are read back
 // from LDS instead of being carried in registers, and each wave keeps its
-// inner x-fluxes there for the three rows until they are averaged.  The
-// interior kernel then fits 168 VGPRs, three workgroups per CU.
+// inner x-fluxes there for the three rows until they are averaged: fewer
+// VGPRs, more waves per CU.
 constexpr int LR_DEPTH = 8;  // slots: rows t-4 .. t+3 at row t
 constexpr int LR_AHEAD = 3;  // rows issued ahead of the row computed
 constexpr int LR_NL = 12;  // area cyu cxu mfy mfx rarea gyp gym gxp gxm dp1.lo dp1.hi
@@ -690,27 +694,38 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
             ring_issue<W, NT, EK>(a, sc, st, ring, wave, t);
         }
       };
+      // The lean kind runs whole groups of six rows: the rows past the block
+      // read ring slots that were not reloaded (in-bounds LDS) and store
+      // nothing (x.out_row is false there), and the loop carries no per-row
+      // guard.  Every wave of the workgroup steps through the same rows, so
+      // the barriers in ring_wait pair up.
+      const int nrun = EK == 0 ? (nrow + 5) / 6 * 6 : nrow;
       for (int t = 0; t < LR_AHEAD && t < nrow; t++) issue(t);
 #define FV3_RROW(U, TW)                                                              \
-  if (t0 + U < nrow) {                                                               \
+  if (EK == 0 || t0 + U < nrow) {                                                    \
     const int t = t0 + U;                                                            \
     ring_wait<W, NT, QN>(min(LR_AHEAD - 1, nrow - 1 - t));                           \
     if (t + LR_AHEAD < nrow) issue(t + LR_AHEAD);                                    \
-    if (any) {                                                                       \
-      RowIn<T, NT> in;                                                               \
-      ring_read<W, NT, QN>(ring, wave, lane, t, in);                                 \
-      row_step<T, NT, U, HO, ek_side(EK), TW>(a, sc, lv, st, act, t, in);            \
-    }                                                                                \
+    RowIn<T, NT> in;                                                                 \
+    ring_read<W, NT, QN>(ring, wave, lane, t, in);                                   \
+    row_step<T, NT, U, HO, ek_side(EK), TW>(a, sc, lv, st, act, t, in);              \
   }
-      // the lean kind runs the block's first six input rows (the y-pipes'
-      // warm-up) as their own unrolled group; the edge kinds, held to 168
-      // VGPRs by their occupancy, would spill with it
-      if constexpr (EK == 0) {
-        constexpr int t0 = 0;
-        FV3_RROW(0, 0) FV3_RROW(1, 1) FV3_RROW(2, 2) FV3_RROW(3, 3) FV3_RROW(4, 4) FV3_RROW(5, 5)
-      }
-      for (int t0 = EK == 0 ? 6 : 0; t0 < nrow; t0 += 6) {
-        FV3_RROW(0, 6) FV3_RROW(1, 6) FV3_RROW(2, 6) FV3_RROW(3, 6) FV3_RROW(4, 6) FV3_RROW(5, 6)
+      if (any) {
+        // the lean kind runs the block's first six input rows (the y-pipes'
+        // warm-up) as their own unrolled group; the edge kinds, held to 168
+        // VGPRs by their occupancy, would spill with it
+        if constexpr (EK == 0) {
+          constexpr int t0 = 0;
+          FV3_RROW(0, 0) FV3_RROW(1, 1) FV3_RROW(2, 2) FV3_RROW(3, 3) FV3_RROW(4, 4) FV3_RROW(5, 5)
+        }
+        for (int t0 = EK == 0 ? 6 : 0; t0 < nrun; t0 += 6) {
+          FV3_RROW(0, 6) FV3_RROW(1, 6) FV3_RROW(2, 6) FV3_RROW(3, 6) FV3_RROW(4, 6) FV3_RROW(5, 6)
+        }
+      } else {  // no tracer of this wave in this round: its loads and barriers only
+        for (int t = 0; t < nrun; t++) {
+          ring_wait<W, NT, QN>(min(LR_AHEAD - 1, nrow - 1 - t));
+          if (t + LR_AHEAD < nrow) issue(t + LR_AHEAD);
+        }
       }
 #undef FV3_RROW
       continue;

@@ -46,10 +46,11 @@ struct GridDev {
 };
 
 // FP32 fused launch configurations (waves per workgroup x tracers per wave
-// per kernel kind); the default: 4 x 2 for the interior kernel and 4 x 1 for
-// the edge kernels, all at three workgroups per CU
+// per kernel kind); the default: 8 x 2 for the interior kernel (two
+// workgroups, 16 waves per CU) and 4 x 1 at three workgroups per CU for the
+// edge kernels
 constexpr int FUSED32_NCFG = 15;
-constexpr int FUSED32_DEFAULT_CFG = 11;
+constexpr int FUSED32_DEFAULT_CFG = 12;
 // objects the FP32 configurations are compiled into (cfg % FUSED32_NPART)
 constexpr int FUSED32_NPART = 6;
 

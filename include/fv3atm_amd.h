@@ -166,9 +166,9 @@ int fv3_step_dev(fv3_ctx *ctx, int form, int nq, const void *q_in, void *q_out, 
 int fv3_ctx_set_timing(fv3_ctx *ctx, int on);
 /* Kernel choices of the FP32 step (no counterpart upstream): cfg selects
  * the fused transport launch shape, waves per workgroup x tracers per wave
- * (0..14; the default is 11: 4 waves x 2 tracers for the interior kernel, 4 x 1
- * for the edge kernels, all at three workgroups per CU); rows the interior
- * rows per block (0 = 60);
+ * (0..14; the default is 12: 8 waves x 2 tracers for the interior kernel, 4 x 1
+ * at three workgroups per CU for the edge kernels); rows the interior rows per
+ * block (0 = 60);
  * remap the vertical remap kernel: 0 (default) = column-shared, a column's
  * levels spread over 16 or 32 lanes of a wave (npz <= 127; larger npz take
  * kernel 2), 1 = one thread per column, 2 = column-shared, one thread per
