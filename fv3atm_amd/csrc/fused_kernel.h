@@ -694,15 +694,17 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
             ring_issue<W, NT, EK>(a, sc, st, ring, wave, t);
         }
       };
-      // The lean kind runs whole groups of six rows: the rows past the block
-      // read ring slots that were not reloaded (in-bounds LDS) and store
-      // nothing (x.out_row is false there), and the loop carries no per-row
-      // guard.  Every wave of the workgroup steps through the same rows, so
-      // the barriers in ring_wait pair up.
-      const int nrun = EK == 0 ? (nrow + 5) / 6 * 6 : nrow;
+      // The lean kinds (0, and 5 / 6 in the edge strips, whose blocks are H
+      // rows) run whole groups of six rows: the rows past the block read ring
+      // slots that were not reloaded (in-bounds LDS) and store nothing
+      // (x.out_row is false there), and the loop carries no per-row guard.
+      // Every wave of the workgroup steps through the same rows, so the
+      // barriers in ring_wait pair up.
+      constexpr bool PAD = EK == 0 || EK >= 5;
+      const int nrun = PAD ? (nrow + 5) / 6 * 6 : nrow;
       for (int t = 0; t < LR_AHEAD && t < nrow; t++) issue(t);
 #define FV3_RROW(U, TW)                                                              \
-  if (EK == 0 || t0 + U < nrow) {                                                    \
+  if (PAD || t0 + U < nrow) {                                                        \
     const int t = t0 + U;                                                            \
     ring_wait<W, NT, QN>(min(LR_AHEAD - 1, nrow - 1 - t));                           \
     if (t + LR_AHEAD < nrow) issue(t + LR_AHEAD);                                    \
@@ -712,8 +714,8 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
   }
       if (any) {
         // the lean kind runs the block's first six input rows (the y-pipes'
-        // warm-up) as their own unrolled group; the edge kinds, held to 168
-        // VGPRs by their occupancy, would spill with it
+        // warm-up) as their own unrolled group; the edge kinds would need
+        // more VGPRs with it (kinds 5 and 6: 142 against 95)
         if constexpr (EK == 0) {
           constexpr int t0 = 0;
           FV3_RROW(0, 0) FV3_RROW(1, 1) FV3_RROW(2, 2) FV3_RROW(3, 3) FV3_RROW(4, 4) FV3_RROW(5, 5)
