@@ -172,6 +172,21 @@ def test_step_every_launch_shape(gpu_ready):
             assert rel_l2(interior(got["q"]), interior(base)) <= 1e-6
 
 
+def test_step_two_rounds_default_shape(gpu_ready):
+    # the context's default launch (16 tracers per lean round) over 19 tracers:
+    # a full round, then one with a full wave, a wave with one inactive tracer
+    # and idle waves; 50-row blocks leave a short last block (padded rows)
+    case = cases.make_case("C96L4Q19jet", n=96, npz=4, nq=19, jet=True, courant=2.1, delp_kind=1,
+                           delp_amp=(0.05, 0.2), seed=31, with_pt=True)
+    print("[fused] C96 L4 Q19, default launch, 50 rows per block", flush=True)
+    ref = reference(case)
+    got = run_step(case, fa.FORM_FUSED32, rows=50, cfg=fa.DEFAULT_LAUNCH_CFG)
+    errs = [rel_l2(interior(got["q"][:, :, :, iq]), interior(ref["q"][:, :, :, iq])) for iq in range(case.nq)]
+    print(f"max rel L2 vs oracle {max(errs):.2e}", flush=True)
+    assert max(errs) <= 1e-5
+    assert rel_l2(interior(got["pt"]), interior(ref["pt"])) <= 1e-6
+
+
 @pytest.mark.parametrize("form", [fa.FORM_FUSED32, fa.FORM_FUSED64])
 def test_step_concurrent_kinds_equal_serial(gpu_ready, form):
     # the edge kernels on their own streams give the same bits as run after
