@@ -36,6 +36,8 @@ struct TrState {
   const T *qin, *qhalo;
   const T *qsrc0;  // pass input of tile 1 at this tracer and level (six-tile halos)
   T *qout;
+  __amdgpu_buffer_rsrc_t qout_r;  // FP32: qout as a buffer (scalar row offset, lane offset)
+  __amdgpu_buffer_rsrc_t qin_r;   // FP32: qin likewise
 };
 
 // level data carried between rows
@@ -346,10 +348,14 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     const T Fxp = from_right(Fx);
     if (act[n] && x.out_row && sc.col_out) {
       T *orow = s.qout + x.corow;
-      if constexpr (fp64)
+      if constexpr (fp64) {
         orow[sc.lo_q] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) / dp2;
-      else
-        orow[sc.lo_q] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) * rdp2;
+      } else {
+        const float v = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) * rdp2;
+        (void)orow;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), s.qout_r, sc.lo_q * 4,
+                                              (int)x.corow * 4, 0);
+      }
     }
     s.Fy_prev = Fy;
   }
@@ -445,6 +451,7 @@ struct LeanRows {
   static constexpr int M = (LR_NL + W - 1) / W;
   const float *p[M];
   int step[M], loff[M];
+  __amdgpu_buffer_rsrc_t r[M];  // p[m] as a buffer: row t at scalar offset 4 t step, lane at 4 loff
 };
 
 template <int W>
@@ -476,6 +483,7 @@ __device__ __forceinline__ LeanRows<W> lean_rows(const FusedArgs<float> &a, cons
     lr.p[m] = p;
     lr.step[m] = step;
     lr.loff[m] = loff;
+    lr.r[m] = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, 0x7fffffff, 0x00020000);
   }
   return lr;
 }
@@ -487,11 +495,14 @@ __device__ __forceinline__ void ring_issue_lean(const FusedArgs<float> &a, const
   float(*slot)[FT_LANES] = ring.v[t & (LR_DEPTH - 1)];
 #pragma unroll
   for (int m = 0; m < LeanRows<W>::M; m++)
-    if (wave + m * W < LR_NL) lds_dma(lr.p[m] + (long long)t * lr.step[m] + lr.loff[m], slot[wave + m * W]);
-  const long long orow = (long long)(sc.j0 - NG + t - a.jsd) * a.nid;
+    if (wave + m * W < LR_NL)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(lr.r[m], (__attribute__((address_space(3))) void *)slot[wave + m * W], 4,
+                                               lr.loff[m] * 4, t * lr.step[m] * 4, 0, 0);
+  const int orow = (sc.j0 - NG + t - a.jsd) * a.nid;
 #pragma unroll
   for (int n = 0; n < NT; n++)  // inactive tracers load tracer 0: every wave issues the same count
-    lds_dma(st[n].qin + orow + sc.lo_q, slot[LR_NL + wave * NT + n]);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(st[n].qin_r, (__attribute__((address_space(3))) void *)slot[LR_NL + wave * NT + n],
+                                             4, sc.lo_q * 4, orow * 4, 0, 0);
 }
 
 template <int W, int NT, int EK>
@@ -666,6 +677,10 @@ __global__ void __launch_bounds__(FT_LANES *W, OCC) k_fused_tp(const FusedArgs<T
       st[n].qin = in + toff + qoff;
       st[n].qhalo = a.q_in + toff + qoff;
       st[n].qout = out + toff + qoff;
+      if constexpr (sizeof(T) == 4) {
+        st[n].qout_r = __builtin_amdgcn_make_buffer_rsrc(st[n].qout, (short)0, 0x7fffffff, 0x00020000);
+        st[n].qin_r = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(st[n].qin), (short)0, 0x7fffffff, 0x00020000);
+      }
       st[n].fy2_prev = st[n].Fy_prev = T(0);
     }
     LvState<T> lv;
