@@ -513,7 +513,9 @@ __device__ __forceinline__ void ring_issue_mixed(const FusedArgs<float> &a, cons
   float(*slot)[FT_LANES] = ring.v[t & (LR_DEPTH - 1)];
 #pragma unroll
   for (int m = 0; m < LeanRows<W>::M; m++)
-    if (wave + m * W < LR_NL) lds_dma(lr.p[m] + (long long)t * lr.step[m] + lr.loff[m], slot[wave + m * W]);
+    if (wave + m * W < LR_NL)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(lr.r[m], (__attribute__((address_space(3))) void *)slot[wave + m * W], 4,
+                                               lr.loff[m] * 4, t * lr.step[m] * 4, 0, 0);
   const RowIdx<float> x = row_idx(a, sc, t);
 #pragma unroll
   for (int n = 0; n < NT; n++) {
