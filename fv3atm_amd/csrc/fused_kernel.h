@@ -347,12 +347,10 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
     const T Fx = T(0.5) * (fx + (fp64 ? s.fx2_r[M3] : in.fx2m[n])) * mfxs;
     const T Fxp = from_right(Fx);
     if (act[n] && x.out_row && sc.col_out) {
-      T *orow = s.qout + x.corow;
       if constexpr (fp64) {
-        orow[sc.lo_q] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) / dp2;
-      } else {
+        (s.qout + x.corow)[sc.lo_q] = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) / dp2;
+      } else {  // the row as the scalar offset, the lane's column as the vector offset
         const float v = (qm * d1 + (Fx - Fxp + s.Fy_prev - Fy) * rarea) * rdp2;
-        (void)orow;
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), s.qout_r, sc.lo_q * 4,
                                               (int)x.corow * 4, 0);
       }
@@ -364,7 +362,8 @@ __device__ __forceinline__ void row_step(const FusedArgs<T> &a, const StripCtx<T
 }
 
 // ---- LDS row ring of the FP32 kernels.  Rows are loaded straight into LDS
-// with global_load_lds_dword (the data in flight holds no VGPRs), so the
+// with global_load_lds_dword, or buffer_load_dword ... lds where the row
+// address is a scalar offset (the data in flight holds no VGPRs), so the
 // loads of a row can be issued LR_AHEAD rows ahead.  The waves of a
 // workgroup split the LR_NL level arrays of a row between them (they stream
 // the same strip and level) and each loads its own tracers' rows.  The ring
